@@ -1,0 +1,417 @@
+// fss_api.cpp -- C ABI of libfss (include/fss.h): context, configuration,
+// interpolation constants and argument checking.  No exception crosses the ABI.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+
+#include "fss_internal.h"
+
+namespace fss {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+fss_status DevBuf::ensure(size_t nbytes)
+{
+    if (nbytes <= bytes && p) return FSS_OK;
+    release();
+    if (nbytes == 0) nbytes = 16;
+    hipError_t e = hipMalloc(&p, nbytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        bytes = 0;
+        set_error(std::string("hipMalloc(") + std::to_string(nbytes) + "): " + hipGetErrorString(e));
+        return FSS_ERR_NOMEM;
+    }
+    bytes = nbytes;
+    return FSS_OK;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+// ---- interpolation constants (interp_utils.h:12-75) ----
+
+// netlib-order LU (dgetrf2 for small n), column-major; returns info
+static int lu_factor(int n, double *a, int32_t *ipiv)
+{
+    int info = 0;
+    for (int j = 0; j < n; j++) {
+        int jp = j;
+        double amax = std::fabs(a[j + j * n]);
+        for (int i = j + 1; i < n; i++) {
+            double v = std::fabs(a[i + j * n]);
+            if (v > amax) {
+                amax = v;
+                jp = i;
+            }
+        }
+        ipiv[j] = jp;
+        if (a[jp + j * n] != 0.0) {
+            if (jp != j)
+                for (int k = 0; k < n; k++) std::swap(a[j + k * n], a[jp + k * n]);
+            double piv = a[j + j * n];
+            if (std::fabs(piv) >= DBL_MIN) {
+                double r = 1.0 / piv;
+                for (int i = j + 1; i < n; i++) a[i + j * n] *= r;
+            } else {
+                for (int i = j + 1; i < n; i++) a[i + j * n] = a[i + j * n] / piv;
+            }
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        for (int k = j + 1; k < n; k++) {
+            double u = a[j + k * n];
+            if (u != 0.0) {
+                double t = -u;
+                for (int i = j + 1; i < n; i++) a[i + k * n] = a[i + k * n] + a[i + j * n] * t;
+            }
+        }
+    }
+    return info;
+}
+
+static void lu_solve(int n, const double *a, const int32_t *ipiv, double *b)
+{
+    for (int k = 0; k < n; k++)
+        if (ipiv[k] != k) std::swap(b[k], b[ipiv[k]]);
+    for (int k = 0; k < n; k++)
+        if (b[k] != 0.0)
+            for (int i = k + 1; i < n; i++) b[i] = b[i] - b[k] * a[i + k * n];
+    for (int k = n - 1; k >= 0; k--)
+        if (b[k] != 0.0) {
+            b[k] = b[k] / a[k + k * n];
+            for (int i = 0; i < k; i++) b[i] = b[i] - b[k] * a[i + k * n];
+        }
+}
+
+// fekete_init (interp_utils.h:12-31): rule 0 runs it as written with the
+// uninitialised b taken as 0; rule 1 is the intended b = 1 - a - c.
+static void interp_nodes(int degree, int rule, double *pts)
+{
+    double delta_x = 1.0 / degree;
+    double a, b = 0.0, c;
+    for (int i = 0; i < degree + 1; i++) {
+        a = 1 - i * delta_x;
+        double a0 = a;
+        for (int j = 0; j < i + 1; j++) {
+            int index = i * (i + 1) / 2 + j;
+            c = j * delta_x;
+            if (rule == FSS_NODES_REFERENCE) {
+                b = 1 - a - b;
+            } else {
+                a = a0;
+                b = 1 - a - c;
+            }
+            a = 0.5 * (1 + std::sin(M_PI / 2 * (2 * a - 1)));
+            b = 0.5 * (1 + std::sin(M_PI / 2 * (2 * b - 1)));
+            c = 0.5 * (1 + std::sin(M_PI / 2 * (2 * c - 1)));
+            pts[3 * index + 0] = a / (a + b + c);
+            pts[3 * index + 1] = b / (a + b + c);
+            pts[3 * index + 2] = c / (a + b + c);
+        }
+    }
+}
+
+fss_status interp_setup(fss_ctx *ctx)
+{
+    Interp &I = ctx->interp;
+    I.degree = ctx->cfg.degree;
+    I.p = (I.degree + 1) * (I.degree + 2) / 2;
+    const int p = I.p;
+    I.pts.assign(3 * p, 0.0);
+    interp_nodes(I.degree, ctx->cfg.node_rule, I.pts.data());
+    // interp_mat_init (interp_utils.h:33-47): A[k + idx*p] = s_k^(i-j) t_k^j
+    I.lu.assign(p * p, 0.0);
+    for (int i = 0; i < I.degree + 1; i++)
+        for (int j = 0; j < i + 1; j++) {
+            int index = i * (i + 1) / 2 + j;
+            for (int k = 0; k < p; k++)
+                I.lu[index * p + k] = std::pow(I.pts[3 * k], i - j) * std::pow(I.pts[3 * k + 1], j);
+        }
+    I.ipiv.assign(p, 0);
+    int info = lu_factor(p, I.lu.data(), I.ipiv.data());
+    if (info != 0) {
+        set_error("interpolation node set is not unisolvent (dgetrf info " + std::to_string(info) + ")");
+        return FSS_ERR_NUMERIC;
+    }
+    I.vinv.assign(p * p, 0.0);
+    for (int k = 0; k < p; k++) {
+        double *col = I.vinv.data() + k * p;
+        col[k] = 1.0;
+        lu_solve(p, I.lu.data(), I.ipiv.data(), col);
+    }
+    hipStream_t s = ctx->stream;
+    FSS_TRY(ctx->d_lu.ensure(sizeof(double) * p * p));
+    FSS_TRY(ctx->d_vinv.ensure(sizeof(double) * p * p));
+    FSS_TRY(ctx->d_ipiv.ensure(sizeof(int32_t) * p));
+    FSS_TRY(ctx->d_pts.ensure(sizeof(double) * 3 * p));
+    FSS_HIP(hipMemcpyAsync(ctx->d_lu.p, I.lu.data(), sizeof(double) * p * p, hipMemcpyHostToDevice, s));
+    FSS_HIP(hipMemcpyAsync(ctx->d_vinv.p, I.vinv.data(), sizeof(double) * p * p, hipMemcpyHostToDevice, s));
+    FSS_HIP(hipMemcpyAsync(ctx->d_ipiv.p, I.ipiv.data(), sizeof(int32_t) * p, hipMemcpyHostToDevice, s));
+    FSS_HIP(hipMemcpyAsync(ctx->d_pts.p, I.pts.data(), sizeof(double) * 3 * p, hipMemcpyHostToDevice, s));
+    FSS_HIP(hipStreamSynchronize(s));
+    return FSS_OK;
+}
+
+static fss_status validate(const fss_config *c)
+{
+    if (!(c->theta > 0) || !std::isfinite(c->theta)) return set_error("theta must be > 0"), FSS_ERR_INVALID;
+    if (c->degree < 1 || c->degree > FSS_MAX_DEGREE)
+        return set_error("degree must be in [1, " + std::to_string(FSS_MAX_DEGREE) + "]"), FSS_ERR_INVALID;
+    if (c->many_count < 0) return set_error("many_count must be >= 0"), FSS_ERR_INVALID;
+    if (c->tree_levels < 1 || c->tree_levels > 13)
+        return set_error("tree_levels must be in [1, 13]"), FSS_ERR_INVALID;
+    if (c->radius != 1.0) return set_error("only the unit sphere is supported (radius = 1)"), FSS_ERR_INVALID;
+    if (c->precision != FSS_FP64 && c->precision != FSS_MIXED)
+        return set_error("precision must be FSS_FP64 or FSS_MIXED"), FSS_ERR_INVALID;
+    if (c->assign_mode != FSS_ASSIGN_PARITY && c->assign_mode != FSS_ASSIGN_ROBUST)
+        return set_error("bad assign_mode"), FSS_ERR_INVALID;
+    if (c->eval_mode != FSS_EVAL_PARITY && c->eval_mode != FSS_EVAL_BLTC)
+        return set_error("bad eval_mode"), FSS_ERR_INVALID;
+    if (c->node_rule != FSS_NODES_REFERENCE && c->node_rule != FSS_NODES_SIN2)
+        return set_error("bad node_rule"), FSS_ERR_INVALID;
+    return FSS_OK;
+}
+
+static bool bad_particles(const fss_particles *p)
+{
+    return !p || !p->x || !p->y || !p->z || !p->vort || !p->area || p->n < 1;
+}
+
+static bool bad_velocity(const fss_velocity *o) { return !o || !o->u || !o->v || !o->w || !o->dvort; }
+
+} // namespace fss
+
+using namespace fss;
+
+#define FSS_GUARD(...)                                                                \
+    try {                                                                              \
+        __VA_ARGS__                                                                    \
+    } catch (const std::bad_alloc &) {                                                 \
+        set_error("host allocation failed");                                           \
+        return FSS_ERR_NOMEM;                                                          \
+    } catch (const std::exception &e) {                                                \
+        set_error(e.what());                                                           \
+        return FSS_ERR_INVALID;                                                        \
+    }
+
+extern "C" {
+
+int fss_version(void) { return 10000; }
+
+const char *fss_last_error(void) { return g_err.c_str(); }
+
+const char *fss_status_string(fss_status s)
+{
+    switch (s) {
+    case FSS_OK: return "ok";
+    case FSS_ERR_INVALID: return "invalid argument";
+    case FSS_ERR_HIP: return "HIP runtime error";
+    case FSS_ERR_NOMEM: return "out of memory";
+    case FSS_ERR_STATE: return "invalid call order";
+    case FSS_ERR_NUMERIC: return "numerical failure";
+    case FSS_ERR_NODEVICE: return "no HIP device";
+    }
+    return "unknown status";
+}
+
+fss_status fss_config_default(fss_config *cfg)
+{
+    if (!cfg) return set_error("null config"), FSS_ERR_INVALID;
+    cfg->theta = 0.7;        // struct_list.h:80
+    cfg->degree = 2;         // new_dynamics.cpp:79
+    cfg->many_count = 10;    // new_dynamics.cpp:78
+    cfg->tree_levels = 1;    // struct_list.h:79
+    cfg->radius = 1.0;       // struct_list.h:58
+    cfg->omega = 2 * M_PI;   // new_dynamics.cpp:43
+    cfg->precision = FSS_FP64;
+    cfg->assign_mode = FSS_ASSIGN_PARITY;
+    cfg->eval_mode = FSS_EVAL_PARITY;
+    cfg->node_rule = FSS_NODES_REFERENCE;
+    cfg->device = 0;
+    return FSS_OK;
+}
+
+fss_status fss_create(fss_ctx **out, const fss_config *cfg)
+{
+    FSS_GUARD({
+        if (!out || !cfg) return set_error("null argument"), FSS_ERR_INVALID;
+        *out = nullptr;
+        FSS_TRY(validate(cfg));
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+            return set_error("no HIP device visible"), FSS_ERR_NODEVICE;
+        if (cfg->device < 0 || cfg->device >= ndev) return set_error("device out of range"), FSS_ERR_INVALID;
+        FSS_HIP(hipSetDevice(cfg->device));
+        fss_ctx *c = new fss_ctx();
+        c->cfg = *cfg;
+        if (hipStreamCreate(&c->stream) != hipSuccess) {
+            delete c;
+            return set_error("hipStreamCreate failed"), FSS_ERR_HIP;
+        }
+        c->own_stream = true;
+        for (auto &e : c->timing.ev) {
+            if (hipEventCreate(&e) != hipSuccess) {
+                fss_destroy(c);
+                return set_error("hipEventCreate failed"), FSS_ERR_HIP;
+            }
+        }
+        fss_status s = interp_setup(c);
+        if (s != FSS_OK) {
+            fss_destroy(c);
+            return s;
+        }
+        *out = c;
+        return FSS_OK;
+    })
+}
+
+fss_status fss_destroy(fss_ctx *ctx)
+{
+    if (!ctx) return FSS_OK;
+    (void)hipSetDevice(ctx->cfg.device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    for (auto &e : ctx->timing.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx; // DevBuf members free their device memory
+    return FSS_OK;
+}
+
+fss_status fss_set_stream(fss_ctx *ctx, void *stream)
+{
+    if (!ctx) return set_error("null context"), FSS_ERR_INVALID;
+    if (ctx->own_stream && ctx->stream) {
+        FSS_HIP(hipStreamSynchronize(ctx->stream));
+        FSS_HIP(hipStreamDestroy(ctx->stream));
+    }
+    if (stream) {
+        ctx->stream = static_cast<hipStream_t>(stream);
+        ctx->own_stream = false;
+    } else {
+        FSS_HIP(hipStreamCreate(&ctx->stream));
+        ctx->own_stream = true;
+    }
+    if (ctx->graph_exec) {
+        (void)hipGraphExecDestroy(ctx->graph_exec);
+        ctx->graph_exec = nullptr;
+    }
+    return FSS_OK;
+}
+
+fss_status fss_sync(fss_ctx *ctx)
+{
+    if (!ctx) return set_error("null context"), FSS_ERR_INVALID;
+    FSS_HIP(hipStreamSynchronize(ctx->stream));
+    return FSS_OK;
+}
+
+fss_status fss_tree_build(fss_ctx *ctx, const double *x, const double *y, const double *z, int64_t n)
+{
+    FSS_GUARD({
+        if (!ctx || !x || !y || !z) return set_error("null argument"), FSS_ERR_INVALID;
+        FSS_HIP(hipSetDevice(ctx->cfg.device));
+        if (ctx->graph_exec) {
+            (void)hipGraphExecDestroy(ctx->graph_exec);
+            ctx->graph_exec = nullptr;
+        }
+        return tree_build(ctx, x, y, z, n);
+    })
+}
+
+fss_status fss_tree_stats_get(fss_ctx *ctx, fss_tree_stats *out)
+{
+    if (!ctx || !out) return set_error("null argument"), FSS_ERR_INVALID;
+    if (!ctx->tree.built) return set_error("no tree"), FSS_ERR_STATE;
+    *out = ctx->tree.stats;
+    return FSS_OK;
+}
+
+fss_status fss_tree_export(fss_ctx *ctx, int32_t *loc, uint8_t *found, int32_t *counts,
+                           int32_t *offsets, int32_t *ids, int32_t *inter, double *tri_info,
+                           double *tri_xyz)
+{
+    FSS_GUARD({
+        if (!ctx) return set_error("null context"), FSS_ERR_INVALID;
+        return tree_export(ctx, loc, found, counts, offsets, ids, inter, tri_info, tri_xyz);
+    })
+}
+
+fss_status fss_velocity_direct(fss_ctx *ctx, const fss_particles *p, fss_velocity *out)
+{
+    FSS_GUARD({
+        if (!ctx || bad_particles(p) || bad_velocity(out)) return set_error("null argument"), FSS_ERR_INVALID;
+        return velocity_direct(ctx, p, nullptr, p->n, out, false);
+    })
+}
+
+fss_status fss_velocity_tree(fss_ctx *ctx, const fss_particles *p, fss_velocity *out)
+{
+    FSS_GUARD({
+        if (!ctx || bad_particles(p) || bad_velocity(out)) return set_error("null argument"), FSS_ERR_INVALID;
+        return velocity_tree(ctx, p, out);
+    })
+}
+
+fss_status fss_velocity_direct_targets(fss_ctx *ctx, const fss_particles *p, const int64_t *idx,
+                                       int64_t m, fss_velocity *out)
+{
+    FSS_GUARD({
+        if (!ctx || bad_particles(p) || bad_velocity(out) || !idx || m < 1)
+            return set_error("null argument or m < 1"), FSS_ERR_INVALID;
+        return velocity_direct(ctx, p, idx, m, out, true);
+    })
+}
+
+fss_status fss_rk4_step(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
+                        const double *area, int64_t n, double dt, int use_tree)
+{
+    FSS_GUARD({
+        if (!ctx || !x || !y || !z || !vort || !area || n < 1) return set_error("null argument"), FSS_ERR_INVALID;
+        return rk4_step(ctx, x, y, z, vort, area, n, dt, use_tree);
+    })
+}
+
+fss_status fss_rk4_run_graph(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
+                             const double *area, int64_t n, double dt, int use_tree, int steps)
+{
+    FSS_GUARD({
+        if (!ctx || !x || !y || !z || !vort || !area || n < 1) return set_error("null argument"), FSS_ERR_INVALID;
+        return rk4_run_graph(ctx, x, y, z, vort, area, n, dt, use_tree, steps);
+    })
+}
+
+fss_status fss_project_points(fss_ctx *ctx, double *x, double *y, double *z, double *vort, int64_t n)
+{
+    if (!ctx || !x || !y || !z || !vort || n < 1) return set_error("null argument"), FSS_ERR_INVALID;
+    return project_points(ctx, x, y, z, vort, n);
+}
+
+fss_status fss_last_timing(fss_ctx *ctx, double *stage_ms, double *main_ms)
+{
+    if (!ctx) return set_error("null context"), FSS_ERR_INVALID;
+    if (!ctx->timing.valid) return set_error("no timed call yet"), FSS_ERR_STATE;
+    hipEvent_t *ev = ctx->timing.ev;
+    FSS_HIP(hipEventSynchronize(ev[3]));
+    float a = 0, b = 0;
+    FSS_HIP(hipEventElapsedTime(&a, ev[0], ev[3]));
+    FSS_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+    if (stage_ms) *stage_ms = a;
+    if (main_ms) *main_ms = b;
+    return FSS_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,190 @@
+// fss_internal.h -- context layout and kernel entry points shared by the
+// libfss translation units.  Not part of the public ABI (include/fss.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fss.h"
+
+// Largest supported interpolation degree and proxy count (p = (d+1)(d+2)/2).
+#define FSS_MAX_DEGREE 8
+#define FSS_MAX_P 45
+
+namespace fss {
+
+// Thread-local last error message for fss_last_error().
+void set_error(const std::string &msg);
+
+#define FSS_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::fss::set_error(std::string(#call) + ": " + hipGetErrorString(e_));       \
+            return FSS_ERR_HIP;                                                         \
+        }                                                                               \
+    } while (0)
+
+#define FSS_TRY(call)                                                                   \
+    do {                                                                                \
+        fss_status s_ = (call);                                                         \
+        if (s_ != FSS_OK) return s_;                                                    \
+    } while (0)
+
+// Growable device buffer (move-only, freed on destruction).
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    // grows (contents are not preserved) so that at least nbytes are usable
+    fss_status ensure(size_t nbytes);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// One work item of the direct-target ("near") kernel: up to 64 targets of one
+// node, a list of source ranges, and an optional interpolant to add.
+struct NearItem {
+    int32_t tgt_start; // position in the level-ordered source array
+    int32_t tgt_cnt;   // 1..64
+    int32_t a_beg;     // [a_beg, a_end) into the sorted range list of class A
+    int32_t a_end;
+    int32_t alpha_slot; // >= 0: add interpolant alpha[slot] (class-B work of this node)
+    int32_t node;       // global node index (geometry of the target triangle)
+};
+
+// One work item of the proxy kernel: the p proxies of one node.
+struct ProxyItem {
+    int32_t node;
+    int32_t b_beg; // [b_beg, b_end) into the sorted range list of class B
+    int32_t b_end;
+    int32_t slot; // alpha output slot
+};
+
+// Interpolation constants (host copy + device copy).
+struct Interp {
+    int degree = 2, p = 6;
+    std::vector<double> pts;  // p x 3 barycentric nodes
+    std::vector<double> lu;   // p x p column-major LU of the Vandermonde
+    std::vector<int32_t> ipiv;
+    std::vector<double> vinv; // p x p column-major inverse Vandermonde
+};
+
+struct Tree {
+    bool built = false;
+    int L = 0;            // tree levels used (0..L-1)
+    int64_t n = 0;        // particles
+    int64_t nn = 0;       // nodes on levels 0..L-1
+    int64_t ntri_all = 0; // triangles on levels 0..L (geometry is built one level deeper)
+    int64_t p0 = 0;       // level-ordered particle positions (sum of list lengths)
+    int64_t ninter = 0;
+    int ngen = 0;
+    std::vector<int64_t> level_base; // L+1 entries, host copy
+    fss_tree_stats stats{};
+
+    DevBuf tri_xyz;  // 9 doubles per triangle (vertex coordinates, tree_tri_verts order)
+    DevBuf tri_info; // 4 doubles per triangle (centre, radius)
+    DevBuf loc;      // int32 [L][n]
+    DevBuf found;    // uint8 [L][n]
+    DevBuf counts;   // int32 [nn]
+    DevBuf offsets;  // int32 [nn] global start in ids
+    DevBuf ids;      // int32 [p0]
+    DevBuf inter;    // int32 [7][ninter] AoS
+    DevBuf inv_active; // int32 [L][n]: position of particle in an active node or -1
+
+    // evaluation plan
+    int64_t n_near = 0, n_proxy = 0, n_ra = 0, n_rb = 0, n_charge = 0;
+    DevBuf near_items;  // NearItem[n_near]
+    DevBuf proxy_items; // ProxyItem[n_proxy]
+    DevBuf ranges_a;    // int2 (start, count) [n_ra]
+    DevBuf ranges_b;    // int2 [n_rb]
+    DevBuf charge_nodes; // int32 [n_charge] nodes whose proxies carry charges (BLTC)
+    DevBuf proxy_xyz;   // double4 [nn * p] static proxy positions (w unused)
+    int64_t pairs_near = 0, pairs_far = 0;
+};
+
+struct Timing {
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool valid = false;
+};
+
+} // namespace fss
+
+struct fss_ctx {
+    fss_config cfg{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    fss::Interp interp;
+    fss::Tree tree;
+    fss::Timing timing;
+
+    // device constants
+    fss::DevBuf d_lu, d_ipiv, d_vinv, d_pts;
+
+    // per-stage workspace
+    fss::DevBuf src64;   // double4 [p0 + nn*p] or [n] for the direct sum
+    fss::DevBuf src32;   // float4
+    fss::DevBuf srcid;   // int32 ids for the direct sum (unused for tree: ids reused)
+    fss::DevBuf out3;    // 3 doubles SoA [p0] per-position partial velocities
+    fss::DevBuf alpha;   // 3*p doubles per proxy item
+    fss::DevBuf rk_k;    // 4 stages x 4 fields x n
+    fss::DevBuf rk_y;    // stage state 4 x n
+    int64_t rk_n = 0;
+
+    // hipGraph of one RK4 step
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    const void *graph_key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int64_t graph_n = 0;
+    double graph_dt = 0;
+    int graph_tree = -1;
+    bool capturing = false; // inside hipStreamBeginCapture: no event records
+};
+
+namespace fss {
+
+// ---- kernels / launchers (fss_tree.hip) ----
+fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const double *z, int64_t n);
+fss_status tree_export(fss_ctx *ctx, int32_t *loc, uint8_t *found, int32_t *counts,
+                       int32_t *offsets, int32_t *ids, int32_t *inter, double *tri_info,
+                       double *tri_xyz);
+void tree_release(Tree &t);
+
+// ---- interpolation constants (fss_api.cpp) ----
+fss_status interp_setup(fss_ctx *ctx);
+
+// ---- evaluation (fss_eval.hip) ----
+fss_status velocity_tree(fss_ctx *ctx, const fss_particles *p, fss_velocity *out);
+
+// ---- direct sum (fss_direct.hip) ----
+fss_status velocity_direct(fss_ctx *ctx, const fss_particles *p, const int64_t *idx, int64_t m,
+                           fss_velocity *out, bool force_fp64);
+
+// ---- RK4 (fss_rk4.hip) ----
+fss_status rk4_step(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
+                    const double *area, int64_t n, double dt, int use_tree);
+fss_status rk4_run_graph(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
+                         const double *area, int64_t n, double dt, int use_tree, int steps);
+fss_status project_points(fss_ctx *ctx, double *x, double *y, double *z, double *vort, int64_t n);
+
+inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+} // namespace fss

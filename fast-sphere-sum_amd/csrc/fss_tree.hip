@@ -1,0 +1,724 @@
+// fss_tree.hip -- icosahedral tree, point location, node lists, dual
+// traversal and evaluation plan, all built on the device.
+//
+// Compiled with -ffp-contract=off: the geometry and the 3x3 solves decide
+// integer tree data and must round exactly as the reference's operation
+// order does (see fss_geom.h).
+//
+// Reference map (Old-BVE-code/):
+//   icos_init            icos_utils.h:15-133   -> k_geom_level0, k_geom_children
+//   points_assign        icos_utils.h:145-190  -> k_locate
+//   tree_tri_points      icos_utils.h:165      -> per-level stable radix sort by node
+//   tree_traverse        fast_rhs.h:16-91      -> k_trav_classify / k_trav_scatter
+//     The reference pops a FIFO queue and appends the 4 children of a split
+//     pair at its tail.  FIFO order processes the queue generation by
+//     generation (every pair of generation g is popped before any pair of
+//     generation g+1), and inside a generation in the order the pairs were
+//     appended.  Each generation is therefore one data-parallel pass:
+//     classify every pair, exclusive-scan the emit flags and the split
+//     counts, scatter.  Emitted pairs land in the list in exactly the
+//     reference's order.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fss_geom.h"
+#include "fss_internal.h"
+
+namespace fss {
+
+using fssg::level_offset;
+using fssg::level_size;
+
+// ------------------------------------------------------------------
+// geometry
+// ------------------------------------------------------------------
+__global__ void k_geom_level0(double *tri_xyz, double *tri_info, double radius)
+{
+    int i = threadIdx.x;
+    if (i >= 20) return;
+    fssg::IcosBase b;
+    fssg::icos_base(&b);
+    double v[3][3];
+    for (int m = 0; m < 3; m++) fssg::project_to_sphere_2(b.v[b.f[i][m] - 1], radius, v[m]);
+    for (int m = 0; m < 3; m++)
+        for (int k = 0; k < 3; k++) tri_xyz[9 * i + 3 * m + k] = v[m][k];
+    double c[3];
+    fssg::tri_center(v[0], v[1], v[2], radius, c);
+    double r = fssg::tri_radius(v[0], v[1], v[2], c);
+    tri_info[4 * i + 0] = c[0];
+    tri_info[4 * i + 1] = c[1];
+    tri_info[4 * i + 2] = c[2];
+    tri_info[4 * i + 3] = r;
+}
+
+__global__ void k_geom_children(double *tri_xyz, double *tri_info, int64_t parent_off,
+                                int64_t child_off, int64_t nparent, double radius)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nparent) return;
+    const double *P = tri_xyz + 9 * (parent_off + j);
+    double v1[3], v2[3], v3[3];
+    for (int k = 0; k < 3; k++) {
+        v1[k] = P[k];
+        v2[k] = P[3 + k];
+        v3[k] = P[6 + k];
+    }
+    double v12[3], v23[3], v31[3];
+    fssg::midpoints(v1, v2, v3, radius, v12, v23, v31);
+    for (int c = 0; c < 4; c++) {
+        const double *tv[3], *cv[3];
+        fssg::child_tri(c, v1, v2, v3, v12, v23, v31, tv, cv);
+        int64_t g = child_off + 4 * j + c;
+        for (int m = 0; m < 3; m++)
+            for (int k = 0; k < 3; k++) tri_xyz[9 * g + 3 * m + k] = tv[m][k];
+        double ce[3];
+        fssg::tri_center(cv[0], cv[1], cv[2], radius, ce);
+        double r = fssg::tri_radius(cv[0], cv[1], cv[2], ce);
+        tri_info[4 * g + 0] = ce[0];
+        tri_info[4 * g + 1] = ce[1];
+        tri_info[4 * g + 2] = ce[2];
+        tri_info[4 * g + 3] = r;
+    }
+}
+
+// ------------------------------------------------------------------
+// point location (icos_utils.h:145-170)
+// ------------------------------------------------------------------
+__global__ void k_locate(const double *__restrict__ x, const double *__restrict__ y,
+                         const double *__restrict__ z, int64_t n, int L,
+                         const double *__restrict__ tri_xyz, int robust, int32_t *loc,
+                         uint8_t *found)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    double p[3] = {x[id], y[id], z[id]};
+    int prev = 0;
+    for (int l = 0; l < L; l++) {
+        int lb = l > 0 ? 4 * prev : 0;
+        int ub = l > 0 ? lb + 4 : 20;
+        int hit = -1, best = lb;
+        double bestmin = -INFINITY;
+        int64_t lo = level_offset(l);
+        for (int j = lb; j < ub; j++) {
+            const double *T = tri_xyz + 9 * (lo + j);
+            double c[3];
+            fssg::barycoords(T, T + 3, T + 6, p, c);
+            if ((c[0] >= 0) && (c[1] >= 0) && (c[2] >= 0)) {
+                hit = j;
+                break;
+            }
+            double mn = c[0] < c[1] ? c[0] : c[1];
+            mn = mn < c[2] ? mn : c[2];
+            if (mn > bestmin) {
+                bestmin = mn;
+                best = j;
+            }
+        }
+        if (hit < 0 && robust) hit = best;
+        // a particle that passes no child keeps location 0 (icos_utils.h:139-140)
+        prev = hit >= 0 ? hit : 0;
+        loc[(int64_t)l * n + id] = prev;
+        found[(int64_t)l * n + id] = hit >= 0 ? 1 : 0;
+    }
+}
+
+__global__ void k_level_keys(const int32_t *loc, const uint8_t *found, int64_t n, uint32_t sentinel,
+                             uint32_t *keys, int32_t *vals, int32_t *counts_level)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    bool f = found[id] != 0;
+    keys[id] = f ? (uint32_t)loc[id] : sentinel;
+    vals[id] = (int32_t)id;
+    if (f) atomicAdd(&counts_level[loc[id]], 1);
+}
+
+// ------------------------------------------------------------------
+// traversal (fast_rhs.h:16-91)
+// ------------------------------------------------------------------
+enum : int8_t { ACT_SKIP = -1, ACT_SPLIT_T = 4, ACT_SPLIT_S = 5 };
+
+__global__ void k_trav_init(int4 *F)
+{
+    int i = threadIdx.x + blockIdx.x * blockDim.x;
+    if (i >= 400) return;
+    F[i] = make_int4(i / 20, i % 20, 0, 0);
+}
+
+__device__ inline int32_t node_count(const int32_t *counts, int lev, int idx)
+{
+    return counts[level_offset(lev) + idx];
+}
+
+__global__ void k_trav_classify(const int4 *__restrict__ F, int64_t nF,
+                                const int32_t *__restrict__ counts,
+                                const double *__restrict__ tri_info, double theta, int many_count,
+                                int L, double radius, int8_t *act, int32_t *eflag, int32_t *scnt,
+                                unsigned long long *margin_bits)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nF) return;
+    int4 q = F[i];
+    int ct = q.x, cs = q.y, lt = q.z, ls = q.w;
+    int32_t nt = node_count(counts, lt, ct), ns = node_count(counts, ls, cs);
+    int8_t a = ACT_SKIP;
+    if (nt != 0 && ns != 0) {
+        const double *ti = tri_info + 4 * (level_offset(lt) + ct);
+        const double *si = tri_info + 4 * (level_offset(ls) + cs);
+        double distance = fssg::great_circ_dist(ti, si, radius);
+        double separation = (ti[3] + si[3]) / distance;
+        if (distance > 0) {
+            double mg = fabs(separation - theta);
+            unsigned long long b = (unsigned long long)__double_as_longlong(mg);
+            atomicMin(margin_bits, b);
+        }
+        if ((distance > 0) && (separation < theta)) {
+            a = (int8_t)((nt > many_count ? 2 : 0) + (ns > many_count ? 1 : 0));
+        } else if ((nt < many_count) && (ns < many_count)) {
+            a = 0;
+        } else if ((lt == L - 1) && (ls == L - 1)) {
+            a = 0;
+        } else if (lt == L - 1) {
+            a = ACT_SPLIT_S;
+        } else if (ls == L - 1) {
+            a = ACT_SPLIT_T;
+        } else if (nt >= ns) {
+            a = ACT_SPLIT_T;
+        } else {
+            a = ACT_SPLIT_S;
+        }
+    }
+    act[i] = a;
+    eflag[i] = (a >= 0 && a <= 3) ? 1 : 0;
+    scnt[i] = (a == ACT_SPLIT_T || a == ACT_SPLIT_S) ? 4 : 0;
+}
+
+__global__ void k_trav_scatter(const int4 *__restrict__ F, int64_t nF, const int8_t *__restrict__ act,
+                               const int32_t *__restrict__ epos, const int32_t *__restrict__ cpos,
+                               const int32_t *__restrict__ counts, int32_t *inter, int64_t ebase,
+                               int4 *Fn)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nF) return;
+    int4 q = F[i];
+    int8_t a = act[i];
+    if (a >= 0 && a <= 3) {
+        int32_t *o = inter + 7 * (ebase + epos[i]);
+        o[0] = q.z;
+        o[1] = q.w;
+        o[2] = q.x;
+        o[3] = q.y;
+        o[4] = node_count(counts, q.z, q.x);
+        o[5] = node_count(counts, q.w, q.y);
+        o[6] = a;
+    } else if (a == ACT_SPLIT_T) {
+        int4 *d = Fn + cpos[i];
+        for (int c = 0; c < 4; c++) d[c] = make_int4(4 * q.x + c, q.y, q.z + 1, q.w);
+    } else if (a == ACT_SPLIT_S) {
+        int4 *d = Fn + cpos[i];
+        for (int c = 0; c < 4; c++) d[c] = make_int4(q.x, 4 * q.y + c, q.z, q.w + 1);
+    }
+}
+
+// ------------------------------------------------------------------
+// evaluation plan
+// ------------------------------------------------------------------
+// class of an interaction: 0 = direct targets (A), 1 = proxy targets (B).
+// PARITY (rhs_eval.h:34-45): pp, pc -> A with particle sources; cp, cc -> B
+// with particle sources.  BLTC: pp -> A particles, pc -> A proxies,
+// cp -> B particles, cc -> B proxies.
+__global__ void k_plan_keys(const int32_t *__restrict__ inter, int64_t m, uint32_t nn,
+                            uint32_t *keys, int32_t *vals, unsigned long long *nA)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int32_t *o = inter + 7 * i;
+    int type = o[6];
+    uint32_t cls = (type == 0 || type == 1) ? 0u : 1u;
+    uint32_t g = (uint32_t)(level_offset(o[0]) + o[2]);
+    keys[i] = cls * nn + g;
+    vals[i] = (int32_t)i;
+    if (cls == 0) atomicAdd(nA, 1ull);
+}
+
+__global__ void k_plan_ranges(const int32_t *__restrict__ inter, const int32_t *__restrict__ sorted_idx,
+                              int64_t m, const int32_t *__restrict__ offsets, int bltc, int p,
+                              int64_t p0, int2 *ranges, unsigned long long *pair_acc)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int32_t *o = inter + 7 * sorted_idx[k];
+    int type = o[6];
+    int64_t gs = level_offset(o[1]) + o[3];
+    bool proxy_src = bltc && (type == 1 || type == 3);
+    int2 r;
+    if (proxy_src) {
+        r.x = (int32_t)(p0 + gs * p);
+        r.y = p;
+    } else {
+        r.x = offsets[gs];
+        r.y = o[5];
+    }
+    ranges[k] = r;
+    // pair accounting: near = direct-target evaluations, far = proxy-target
+    bool clsA = (type == 0 || type == 1);
+    unsigned long long tgt = clsA ? (unsigned long long)o[4] : (unsigned long long)p;
+    atomicAdd(&pair_acc[clsA ? 0 : 1], tgt * (unsigned long long)r.y);
+}
+
+__global__ void k_plan_segments(const uint32_t *__restrict__ keys, int64_t m, uint32_t nn,
+                                int32_t *beg, int32_t *end)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    uint32_t key = keys[k];
+    if (k == 0 || keys[k - 1] != key) beg[key] = (int32_t)k;
+    if (k == m - 1 || keys[k + 1] != key) end[key] = (int32_t)(k + 1);
+}
+
+// per node: number of near chunks and whether it has proxy work
+__global__ void k_plan_nodes(const int32_t *__restrict__ beg, const int32_t *__restrict__ end,
+                             const int32_t *__restrict__ counts, uint32_t nn, int32_t *nchunks,
+                             int32_t *hasB)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nn) return;
+    bool a = end[g] > beg[g];
+    bool b = end[nn + g] > beg[nn + g];
+    int32_t c = counts[g];
+    nchunks[g] = (a || b) ? (int32_t)((c + 63) / 64) : 0;
+    hasB[g] = b ? 1 : 0;
+}
+
+__global__ void k_plan_items(const int32_t *__restrict__ beg, const int32_t *__restrict__ end,
+                             const int32_t *__restrict__ counts, const int32_t *__restrict__ offsets,
+                             const int32_t *__restrict__ chunk_pos, const int32_t *__restrict__ bslot,
+                             const int32_t *__restrict__ hasB, uint32_t nn, int64_t nA,
+                             NearItem *near, ProxyItem *prox)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nn) return;
+    bool a = end[g] > beg[g];
+    bool b = hasB[g] != 0;
+    if (!(a || b)) return;
+    int32_t c = counts[g], off = offsets[g];
+    int32_t slot = b ? bslot[g] : -1;
+    if (b) {
+        ProxyItem it;
+        it.node = (int32_t)g;
+        it.b_beg = (int32_t)(beg[nn + g] - nA);
+        it.b_end = (int32_t)(end[nn + g] - nA);
+        it.slot = slot;
+        prox[slot] = it;
+    }
+    int32_t nch = (c + 63) / 64;
+    for (int32_t q = 0; q < nch; q++) {
+        NearItem it;
+        it.tgt_start = off + 64 * q;
+        it.tgt_cnt = min(64, c - 64 * q);
+        it.a_beg = a ? beg[g] : 0;
+        it.a_end = a ? end[g] : 0;
+        it.alpha_slot = slot;
+        it.node = (int32_t)g;
+        near[chunk_pos[g] + q] = it;
+    }
+}
+
+// inverse map for the final per-particle reduction: for every node with any
+// work, the level-ordered position of each of its particles.
+__global__ void k_plan_inverse(const int32_t *__restrict__ nchunks, const int32_t *__restrict__ counts,
+                               const int32_t *__restrict__ offsets, const int32_t *__restrict__ ids,
+                               uint32_t nn, int L, int64_t n, int32_t *inv_active)
+{
+    // one block per node
+    int64_t g = blockIdx.x;
+    if (g >= nn || nchunks[g] == 0) return;
+    int lev = 0;
+    while (lev + 1 < L && level_offset(lev + 1) <= g) lev++;
+    int32_t c = counts[g], off = offsets[g];
+    for (int32_t t = threadIdx.x; t < c; t += blockDim.x)
+        inv_active[(int64_t)lev * n + ids[off + t]] = off + t;
+}
+
+// nodes whose proxies carry modified charges (BLTC: sources of pc and cc)
+__global__ void k_plan_charge_flags(const int32_t *__restrict__ inter, int64_t m, uint8_t *flag)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int32_t *o = inter + 7 * i;
+    if (o[6] == 1 || o[6] == 3) flag[level_offset(o[1]) + o[3]] = 1;
+}
+
+__global__ void k_plan_compact_flags(const uint8_t *flag, const int32_t *pos, uint32_t nn, int32_t *list)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nn) return;
+    if (flag[g]) list[pos[g]] = (int32_t)g;
+}
+
+// static proxy positions of every node: planar u v1 + v v2 + (1-u-v) v3
+// (fast_rhs.h:188-200) in PARITY, its projection onto the sphere in BLTC.
+__global__ void k_proxy_xyz(const double *__restrict__ tri_xyz, const double *__restrict__ pts,
+                            int p, int64_t nn, int spherical, double4 *proxy)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn * p) return;
+    int64_t g = i / p;
+    int k = (int)(i % p);
+    const double *T = tri_xyz + 9 * g;
+    double u = pts[3 * k], v = pts[3 * k + 1];
+    double q[3];
+    if (!spherical) {
+        for (int c = 0; c < 3; c++) {
+            double p1 = T[c] * u, p2 = T[3 + c] * v, p3 = T[6 + c] * (1.0 - u - v);
+            q[c] = (p1 + p2) + p3;
+        }
+    } else {
+        double w = pts[3 * k + 2];
+        for (int c = 0; c < 3; c++) q[c] = u * T[c] + v * T[3 + c] + w * T[6 + c];
+        double r = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+        for (int c = 0; c < 3; c++) q[c] /= r;
+    }
+    proxy[i] = make_double4(q[0], q[1], q[2], 0.0);
+}
+
+// ------------------------------------------------------------------
+// host orchestration
+// ------------------------------------------------------------------
+static int bits_for(uint64_t maxkey)
+{
+    int b = 1;
+    while (b < 32 && ((uint64_t)1 << b) <= maxkey) b++;
+    return b;
+}
+
+struct Scratch {
+    DevBuf a, b, c, d, e, f, cub;
+};
+
+static fss_status cub_sort_pairs(DevBuf &tmp, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
+                                 int32_t *vout, int64_t n, int bits, hipStream_t s)
+{
+    size_t bytes = 0;
+    FSS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0, bits, s));
+    FSS_TRY(tmp.ensure(bytes));
+    FSS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, kin, kout, vin, vout, (int)n, 0, bits, s));
+    return FSS_OK;
+}
+
+static fss_status cub_exclusive_sum(DevBuf &tmp, const int32_t *in, int32_t *out, int64_t n,
+                                    hipStream_t s)
+{
+    size_t bytes = 0;
+    FSS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s));
+    FSS_TRY(tmp.ensure(bytes));
+    FSS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, (int)n, s));
+    return FSS_OK;
+}
+
+static fss_status d2h(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    FSS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    FSS_HIP(hipStreamSynchronize(s));
+    return FSS_OK;
+}
+
+void tree_release(Tree &t)
+{
+    DevBuf *bufs[] = {&t.tri_xyz, &t.tri_info, &t.loc, &t.found, &t.counts, &t.offsets, &t.ids,
+                      &t.inter, &t.inv_active, &t.near_items, &t.proxy_items, &t.ranges_a,
+                      &t.ranges_b, &t.charge_nodes, &t.proxy_xyz};
+    for (DevBuf *b : bufs) b->release();
+    t.built = false;
+}
+
+fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const double *z, int64_t n)
+{
+    const fss_config &cfg = ctx->cfg;
+    hipStream_t s = ctx->stream;
+    Tree &T = ctx->tree;
+    const int L = cfg.tree_levels;
+    const int p = ctx->interp.p;
+    const bool bltc = cfg.eval_mode == FSS_EVAL_BLTC;
+    if (L < 1 || L > 13) {
+        set_error("tree_levels must be in [1, 13]");
+        return FSS_ERR_INVALID;
+    }
+    if (n < 1 || n > (int64_t)INT32_MAX || (int64_t)L * n > (int64_t)INT32_MAX) {
+        set_error("particle count out of range for 32-bit list offsets");
+        return FSS_ERR_INVALID;
+    }
+    T.built = false;
+    T.L = L;
+    T.n = n;
+    T.nn = level_offset(L);
+    T.ntri_all = level_offset(L + 1);
+    const uint32_t nn = (uint32_t)T.nn;
+    T.stats = fss_tree_stats{};
+    T.stats.n = n;
+    T.stats.tree_levels = L;
+    T.stats.n_nodes = T.nn;
+
+    // ---- geometry: levels 0..L (icos_init builds tree_levels refinements) ----
+    FSS_TRY(T.tri_xyz.ensure(sizeof(double) * 9 * T.ntri_all));
+    FSS_TRY(T.tri_info.ensure(sizeof(double) * 4 * T.ntri_all));
+    hipLaunchKernelGGL(k_geom_level0, dim3(1), dim3(64), 0, s, T.tri_xyz.as<double>(),
+                       T.tri_info.as<double>(), cfg.radius);
+    for (int l = 0; l < L; l++) {
+        int64_t np = level_size(l);
+        hipLaunchKernelGGL(k_geom_children, dim3((unsigned)div_up(np, 256)), dim3(256), 0, s,
+                           T.tri_xyz.as<double>(), T.tri_info.as<double>(), level_offset(l),
+                           level_offset(l + 1), np, cfg.radius);
+    }
+    FSS_HIP(hipGetLastError());
+
+    // ---- point location ----
+    FSS_TRY(T.loc.ensure(sizeof(int32_t) * L * n));
+    FSS_TRY(T.found.ensure((size_t)L * n));
+    hipLaunchKernelGGL(k_locate, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, x, y, z, n, L,
+                       T.tri_xyz.as<double>(), cfg.assign_mode == FSS_ASSIGN_ROBUST ? 1 : 0,
+                       T.loc.as<int32_t>(), T.found.as<uint8_t>());
+    FSS_HIP(hipGetLastError());
+
+    // ---- node lists ----
+    Scratch S;
+    FSS_TRY(T.counts.ensure(sizeof(int32_t) * T.nn));
+    FSS_TRY(T.offsets.ensure(sizeof(int32_t) * T.nn));
+    FSS_HIP(hipMemsetAsync(T.counts.p, 0, sizeof(int32_t) * T.nn, s));
+    FSS_TRY(S.a.ensure(sizeof(uint32_t) * n));
+    FSS_TRY(S.b.ensure(sizeof(uint32_t) * n));
+    FSS_TRY(S.c.ensure(sizeof(int32_t) * n));
+    FSS_TRY(S.d.ensure(sizeof(int32_t) * (size_t)L * n));
+    for (int l = 0; l < L; l++) {
+        hipLaunchKernelGGL(k_level_keys, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s,
+                           T.loc.as<int32_t>() + (int64_t)l * n, T.found.as<uint8_t>() + (int64_t)l * n,
+                           n, (uint32_t)level_size(l), S.a.as<uint32_t>(), S.c.as<int32_t>(),
+                           T.counts.as<int32_t>() + level_offset(l));
+        FSS_TRY(cub_sort_pairs(S.cub, S.a.as<uint32_t>(), S.b.as<uint32_t>(), S.c.as<int32_t>(),
+                               S.d.as<int32_t>() + (int64_t)l * n, n, bits_for((uint64_t)level_size(l)),
+                               s));
+    }
+    FSS_TRY(cub_exclusive_sum(S.cub, T.counts.as<int32_t>(), T.offsets.as<int32_t>(), T.nn, s));
+    // level bases: offsets at the first node of each level, plus the total
+    T.level_base.assign(L + 1, 0);
+    {
+        std::vector<int32_t> o(L), lastc(1), lasto(1);
+        for (int l = 0; l < L; l++)
+            FSS_TRY(d2h(&o[l], T.offsets.as<int32_t>() + level_offset(l), sizeof(int32_t), s));
+        FSS_TRY(d2h(lastc.data(), T.counts.as<int32_t>() + T.nn - 1, sizeof(int32_t), s));
+        FSS_TRY(d2h(lasto.data(), T.offsets.as<int32_t>() + T.nn - 1, sizeof(int32_t), s));
+        for (int l = 0; l < L; l++) T.level_base[l] = o[l];
+        T.level_base[L] = (int64_t)lasto[0] + lastc[0];
+    }
+    T.p0 = T.level_base[L];
+    FSS_TRY(T.ids.ensure(sizeof(int32_t) * std::max<int64_t>(1, T.p0)));
+    for (int l = 0; l < L; l++) {
+        int64_t cnt = T.level_base[l + 1] - T.level_base[l];
+        T.stats.dropped[l] = n - cnt;
+        if (cnt > 0)
+            FSS_HIP(hipMemcpyAsync(T.ids.as<int32_t>() + T.level_base[l], S.d.as<int32_t>() + (int64_t)l * n,
+                                   sizeof(int32_t) * cnt, hipMemcpyDeviceToDevice, s));
+    }
+
+    // ---- traversal ----
+    int64_t capF = 1 << 16, capI = 1 << 16, nF = 400, ninter = 0;
+    DevBuf F0, F1, act, ef, sc, ep, cp, inter, margin;
+    FSS_TRY(F0.ensure(sizeof(int4) * capF));
+    FSS_TRY(inter.ensure(sizeof(int32_t) * 7 * capI));
+    FSS_TRY(margin.ensure(sizeof(unsigned long long)));
+    {
+        double inf = INFINITY;
+        unsigned long long b;
+        std::memcpy(&b, &inf, 8);
+        FSS_HIP(hipMemcpyAsync(margin.p, &b, 8, hipMemcpyHostToDevice, s));
+        FSS_HIP(hipStreamSynchronize(s));
+    }
+    hipLaunchKernelGGL(k_trav_init, dim3(2), dim3(256), 0, s, F0.as<int4>());
+    int gen = 0;
+    while (nF > 0) {
+        FSS_TRY(act.ensure(nF));
+        FSS_TRY(ef.ensure(sizeof(int32_t) * nF));
+        FSS_TRY(sc.ensure(sizeof(int32_t) * nF));
+        FSS_TRY(ep.ensure(sizeof(int32_t) * nF));
+        FSS_TRY(cp.ensure(sizeof(int32_t) * nF));
+        unsigned gb = (unsigned)div_up(nF, 256);
+        hipLaunchKernelGGL(k_trav_classify, dim3(gb), dim3(256), 0, s, F0.as<int4>(), nF,
+                           T.counts.as<int32_t>(), T.tri_info.as<double>(), cfg.theta, cfg.many_count,
+                           L, cfg.radius, act.as<int8_t>(), ef.as<int32_t>(), sc.as<int32_t>(),
+                           margin.as<unsigned long long>());
+        FSS_TRY(cub_exclusive_sum(S.cub, ef.as<int32_t>(), ep.as<int32_t>(), nF, s));
+        FSS_TRY(cub_exclusive_sum(S.cub, sc.as<int32_t>(), cp.as<int32_t>(), nF, s));
+        int32_t tail[4];
+        FSS_TRY(d2h(&tail[0], ep.as<int32_t>() + nF - 1, 4, s));
+        FSS_TRY(d2h(&tail[1], ef.as<int32_t>() + nF - 1, 4, s));
+        FSS_TRY(d2h(&tail[2], cp.as<int32_t>() + nF - 1, 4, s));
+        FSS_TRY(d2h(&tail[3], sc.as<int32_t>() + nF - 1, 4, s));
+        int64_t ne = (int64_t)tail[0] + tail[1], nc = (int64_t)tail[2] + tail[3];
+        if (ninter + ne > capI) {
+            int64_t nc2 = std::max<int64_t>(2 * capI, ninter + ne);
+            DevBuf bigger;
+            FSS_TRY(bigger.ensure(sizeof(int32_t) * 7 * nc2));
+            if (ninter > 0)
+                FSS_HIP(hipMemcpyAsync(bigger.p, inter.p, sizeof(int32_t) * 7 * ninter,
+                                       hipMemcpyDeviceToDevice, s));
+            FSS_HIP(hipStreamSynchronize(s));
+            inter = std::move(bigger);
+            capI = nc2;
+        }
+        FSS_TRY(F1.ensure(sizeof(int4) * std::max<int64_t>(1, nc)));
+        hipLaunchKernelGGL(k_trav_scatter, dim3(gb), dim3(256), 0, s, F0.as<int4>(), nF,
+                           act.as<int8_t>(), ep.as<int32_t>(), cp.as<int32_t>(), T.counts.as<int32_t>(),
+                           inter.as<int32_t>(), ninter, F1.as<int4>());
+        FSS_HIP(hipGetLastError());
+        ninter += ne;
+        std::swap(F0, F1);
+        nF = nc;
+        gen++;
+    }
+    FSS_HIP(hipStreamSynchronize(s));
+    T.ngen = gen;
+    T.ninter = ninter;
+    {
+        unsigned long long b;
+        FSS_TRY(d2h(&b, margin.p, 8, s));
+        std::memcpy(&T.stats.mac_margin, &b, 8);
+    }
+    T.inter = std::move(inter);
+    T.stats.n_interactions = ninter;
+    T.stats.n_generations = gen;
+
+    // type counts
+    if (ninter > 0) {
+        std::vector<int32_t> h(7 * ninter);
+        FSS_TRY(d2h(h.data(), T.inter.p, sizeof(int32_t) * 7 * ninter, s));
+        for (int64_t i = 0; i < ninter; i++) T.stats.n_type[h[7 * i + 6]]++;
+    }
+
+    // ---- evaluation plan ----
+    FSS_TRY(T.proxy_xyz.ensure(sizeof(double4) * T.nn * p));
+    hipLaunchKernelGGL(k_proxy_xyz, dim3((unsigned)div_up(T.nn * p, 256)), dim3(256), 0, s,
+                       T.tri_xyz.as<double>(), ctx->d_pts.as<double>(), p, T.nn, bltc ? 1 : 0,
+                       T.proxy_xyz.as<double4>());
+    int64_t m = ninter;
+    DevBuf keys, keys2, vals, vals2, acc, begend, nch, chpos, hasB, bslot;
+    FSS_TRY(acc.ensure(sizeof(unsigned long long) * 3));
+    FSS_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long) * 3, s));
+    FSS_TRY(begend.ensure(sizeof(int32_t) * 4 * T.nn));
+    FSS_HIP(hipMemsetAsync(begend.p, 0, sizeof(int32_t) * 4 * T.nn, s));
+    int32_t *beg = begend.as<int32_t>(), *end = beg + 2 * T.nn;
+    int64_t nA = 0;
+    if (m > 0) {
+        FSS_TRY(keys.ensure(sizeof(uint32_t) * m));
+        FSS_TRY(keys2.ensure(sizeof(uint32_t) * m));
+        FSS_TRY(vals.ensure(sizeof(int32_t) * m));
+        FSS_TRY(vals2.ensure(sizeof(int32_t) * m));
+        unsigned gb = (unsigned)div_up(m, 256);
+        hipLaunchKernelGGL(k_plan_keys, dim3(gb), dim3(256), 0, s, T.inter.as<int32_t>(), m, nn,
+                           keys.as<uint32_t>(), vals.as<int32_t>(), acc.as<unsigned long long>() + 2);
+        FSS_TRY(cub_sort_pairs(S.cub, keys.as<uint32_t>(), keys2.as<uint32_t>(), vals.as<int32_t>(),
+                               vals2.as<int32_t>(), m, bits_for(2ull * nn), s));
+        unsigned long long hA;
+        FSS_TRY(d2h(&hA, acc.as<unsigned long long>() + 2, 8, s));
+        nA = (int64_t)hA;
+        FSS_TRY(T.ranges_a.ensure(sizeof(int2) * m));
+        hipLaunchKernelGGL(k_plan_ranges, dim3(gb), dim3(256), 0, s, T.inter.as<int32_t>(),
+                           vals2.as<int32_t>(), m, T.offsets.as<int32_t>(), bltc ? 1 : 0, p, T.p0,
+                           T.ranges_a.as<int2>(), acc.as<unsigned long long>());
+        hipLaunchKernelGGL(k_plan_segments, dim3(gb), dim3(256), 0, s, keys2.as<uint32_t>(), m, nn,
+                           beg, end);
+    }
+    T.n_ra = nA;
+    T.n_rb = m - nA;
+    {
+        unsigned long long pa[2];
+        FSS_TRY(d2h(pa, acc.p, 16, s));
+        T.pairs_near = (int64_t)pa[0];
+        T.pairs_far = (int64_t)pa[1];
+        T.stats.pairs_near = T.pairs_near;
+        T.stats.pairs_far = T.pairs_far;
+    }
+    FSS_TRY(nch.ensure(sizeof(int32_t) * T.nn));
+    FSS_TRY(chpos.ensure(sizeof(int32_t) * T.nn));
+    FSS_TRY(hasB.ensure(sizeof(int32_t) * T.nn));
+    FSS_TRY(bslot.ensure(sizeof(int32_t) * T.nn));
+    unsigned gn = (unsigned)div_up(T.nn, 256);
+    hipLaunchKernelGGL(k_plan_nodes, dim3(gn), dim3(256), 0, s, beg, end, T.counts.as<int32_t>(), nn,
+                       nch.as<int32_t>(), hasB.as<int32_t>());
+    FSS_TRY(cub_exclusive_sum(S.cub, nch.as<int32_t>(), chpos.as<int32_t>(), T.nn, s));
+    FSS_TRY(cub_exclusive_sum(S.cub, hasB.as<int32_t>(), bslot.as<int32_t>(), T.nn, s));
+    {
+        int32_t t4[4];
+        FSS_TRY(d2h(&t4[0], chpos.as<int32_t>() + T.nn - 1, 4, s));
+        FSS_TRY(d2h(&t4[1], nch.as<int32_t>() + T.nn - 1, 4, s));
+        FSS_TRY(d2h(&t4[2], bslot.as<int32_t>() + T.nn - 1, 4, s));
+        FSS_TRY(d2h(&t4[3], hasB.as<int32_t>() + T.nn - 1, 4, s));
+        T.n_near = (int64_t)t4[0] + t4[1];
+        T.n_proxy = (int64_t)t4[2] + t4[3];
+    }
+    FSS_TRY(T.near_items.ensure(sizeof(NearItem) * std::max<int64_t>(1, T.n_near)));
+    FSS_TRY(T.proxy_items.ensure(sizeof(ProxyItem) * std::max<int64_t>(1, T.n_proxy)));
+    // class-B ranges follow class-A ranges in the sorted list
+    FSS_TRY(T.ranges_b.ensure(sizeof(int2) * std::max<int64_t>(1, T.n_rb)));
+    if (T.n_rb > 0)
+        FSS_HIP(hipMemcpyAsync(T.ranges_b.p, T.ranges_a.as<int2>() + nA, sizeof(int2) * T.n_rb,
+                               hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_plan_items, dim3(gn), dim3(256), 0, s, beg, end, T.counts.as<int32_t>(),
+                       T.offsets.as<int32_t>(), chpos.as<int32_t>(), bslot.as<int32_t>(),
+                       hasB.as<int32_t>(), nn, nA, T.near_items.as<NearItem>(),
+                       T.proxy_items.as<ProxyItem>());
+    FSS_TRY(T.inv_active.ensure(sizeof(int32_t) * (size_t)L * n));
+    FSS_HIP(hipMemsetAsync(T.inv_active.p, 0xff, sizeof(int32_t) * (size_t)L * n, s));
+    hipLaunchKernelGGL(k_plan_inverse, dim3(nn), dim3(256), 0, s, nch.as<int32_t>(),
+                       T.counts.as<int32_t>(), T.offsets.as<int32_t>(), T.ids.as<int32_t>(), nn, L, n,
+                       T.inv_active.as<int32_t>());
+    // charge nodes (BLTC)
+    T.n_charge = 0;
+    if (bltc && m > 0) {
+        DevBuf flag;
+        FSS_TRY(flag.ensure(T.nn));
+        FSS_HIP(hipMemsetAsync(flag.p, 0, T.nn, s));
+        hipLaunchKernelGGL(k_plan_charge_flags, dim3((unsigned)div_up(m, 256)), dim3(256), 0, s,
+                           T.inter.as<int32_t>(), m, flag.as<uint8_t>());
+        // reuse nch as int flags -> positions
+        std::vector<uint8_t> hf(T.nn);
+        FSS_TRY(d2h(hf.data(), flag.p, T.nn, s));
+        std::vector<int32_t> list;
+        for (int64_t g = 0; g < T.nn; g++)
+            if (hf[g]) list.push_back((int32_t)g);
+        T.n_charge = (int64_t)list.size();
+        FSS_TRY(T.charge_nodes.ensure(sizeof(int32_t) * std::max<int64_t>(1, T.n_charge)));
+        if (T.n_charge > 0)
+            FSS_HIP(hipMemcpyAsync(T.charge_nodes.p, list.data(), sizeof(int32_t) * T.n_charge,
+                                   hipMemcpyHostToDevice, s));
+        FSS_HIP(hipStreamSynchronize(s));
+    }
+    FSS_HIP(hipGetLastError());
+    FSS_HIP(hipStreamSynchronize(s));
+    T.built = true;
+    return FSS_OK;
+}
+
+fss_status tree_export(fss_ctx *ctx, int32_t *loc, uint8_t *found, int32_t *counts, int32_t *offsets,
+                       int32_t *ids, int32_t *inter, double *tri_info, double *tri_xyz)
+{
+    Tree &T = ctx->tree;
+    hipStream_t s = ctx->stream;
+    if (!T.built) {
+        set_error("tree_export before tree_build");
+        return FSS_ERR_STATE;
+    }
+    if (loc) FSS_TRY(d2h(loc, T.loc.p, sizeof(int32_t) * T.L * T.n, s));
+    if (found) FSS_TRY(d2h(found, T.found.p, (size_t)T.L * T.n, s));
+    if (counts) FSS_TRY(d2h(counts, T.counts.p, sizeof(int32_t) * T.nn, s));
+    if (offsets) FSS_TRY(d2h(offsets, T.offsets.p, sizeof(int32_t) * T.nn, s));
+    if (ids && T.p0 > 0) FSS_TRY(d2h(ids, T.ids.p, sizeof(int32_t) * T.p0, s));
+    if (inter && T.ninter > 0) FSS_TRY(d2h(inter, T.inter.p, sizeof(int32_t) * 7 * T.ninter, s));
+    if (tri_info) FSS_TRY(d2h(tri_info, T.tri_info.p, sizeof(double) * 4 * T.ntri_all, s));
+    if (tri_xyz) FSS_TRY(d2h(tri_xyz, T.tri_xyz.p, sizeof(double) * 9 * T.ntri_all, s));
+    return FSS_OK;
+}
+
+} // namespace fss

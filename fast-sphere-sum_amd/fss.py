@@ -1,0 +1,303 @@
+"""Python front end of libfss.so (ctypes over the C ABI in include/fss.h).
+
+Device arrays are torch tensors on a HIP device (float64, contiguous); the
+library runs on torch's current stream.  There is no CPU fallback: if the
+HIP library is missing, `lib()` raises.
+
+Reference functions replaced (Old-BVE-code/):
+  Context.tree_build        icosahedron_init + interp_init + points_assign +
+                            tree_traverse (new_dynamics.cpp:84-88)
+  Context.velocity_tree     rhs_func with use_fast (rhs_eval.h:47-56, 34-45)
+  Context.velocity_direct   rhs_func direct (rhs_eval.h:47-56, 15-32)
+  Context.rk4_step          RK4 loop body (new_dynamics.cpp:142-238)
+  grid_icosahedral etc.     read_points / area_init / vorticity_init inputs
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfss.so")
+
+FSS_OK = 0
+FP64, MIXED = 0, 1
+ASSIGN_PARITY, ASSIGN_ROBUST = 0, 1
+EVAL_PARITY, EVAL_BLTC = 0, 1
+NODES_REFERENCE, NODES_SIN2 = 0, 1
+
+_STATUS = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "out of memory",
+           4: "invalid call order", 5: "numerical failure", 6: "no HIP device"}
+
+
+class FssError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"fss status {status} ({_STATUS.get(status, '?')}): {msg}")
+        self.status = status
+
+
+class Config(C.Structure):
+    _fields_ = [("theta", C.c_double), ("degree", C.c_int), ("many_count", C.c_int),
+                ("tree_levels", C.c_int), ("radius", C.c_double), ("omega", C.c_double),
+                ("precision", C.c_int), ("assign_mode", C.c_int), ("eval_mode", C.c_int),
+                ("node_rule", C.c_int), ("device", C.c_int)]
+
+
+class Particles(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("vort", C.c_void_p),
+                ("area", C.c_void_p), ("n", C.c_int64)]
+
+
+class Velocity(C.Structure):
+    _fields_ = [("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p), ("dvort", C.c_void_p)]
+
+
+class TreeStats(C.Structure):
+    _fields_ = [("n", C.c_int64), ("tree_levels", C.c_int), ("n_nodes", C.c_int64),
+                ("n_interactions", C.c_int64), ("n_type", C.c_int64 * 4),
+                ("dropped", C.c_int64 * 32), ("pairs_near", C.c_int64), ("pairs_far", C.c_int64),
+                ("mac_margin", C.c_double), ("n_generations", C.c_int)]
+
+
+# exported symbol -> (restype, argtypes); the list is the ABI of include/fss.h
+_vp, _i64, _int, _dbl, _u64 = C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_uint64
+SIGNATURES = {
+    "fss_version": (_int, []),
+    "fss_last_error": (C.c_char_p, []),
+    "fss_status_string": (C.c_char_p, [_int]),
+    "fss_config_default": (_int, [C.POINTER(Config)]),
+    "fss_create": (_int, [C.POINTER(_vp), C.POINTER(Config)]),
+    "fss_destroy": (_int, [_vp]),
+    "fss_set_stream": (_int, [_vp, _vp]),
+    "fss_sync": (_int, [_vp]),
+    "fss_tree_build": (_int, [_vp, _vp, _vp, _vp, _i64]),
+    "fss_tree_stats_get": (_int, [_vp, C.POINTER(TreeStats)]),
+    "fss_tree_export": (_int, [_vp] + [_vp] * 8),
+    "fss_velocity_direct": (_int, [_vp, C.POINTER(Particles), C.POINTER(Velocity)]),
+    "fss_velocity_tree": (_int, [_vp, C.POINTER(Particles), C.POINTER(Velocity)]),
+    "fss_velocity_direct_targets": (_int, [_vp, C.POINTER(Particles), _vp, _i64,
+                                           C.POINTER(Velocity)]),
+    "fss_rk4_step": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, _int]),
+    "fss_rk4_run_graph": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, _int, _int]),
+    "fss_project_points": (_int, [_vp, _vp, _vp, _vp, _vp, _i64]),
+    "fss_last_timing": (_int, [_vp, C.POINTER(_dbl), C.POINTER(_dbl)]),
+    "fss_grid_icosahedral": (_int, [_int, _vp, _vp, _vp, _vp]),
+    "fss_grid_voronoi_areas": (_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "fss_grid_third_areas": (_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "fss_grid_random": (_int, [_i64, _u64, _vp, _vp, _vp]),
+    "fss_field_rossby_haurwitz": (_int, [_i64, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp]),
+    "fss_field_gauss_vortices": (_int, [_i64, _vp, _vp, _vp, _vp, _int, _dbl, _dbl, _dbl, _u64,
+                                        _vp]),
+    "fss_sample_targets": (_int, [_i64, _i64, _u64, _vp]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Load libfss.so (built in-tree by __graft_entry__.build()); raise if absent."""
+    global _LIB
+    if _LIB is None:
+        # torch (if installed) ships its own HIP runtime under the same soname;
+        # loading it first makes libfss bind to that one runtime instead of a
+        # second copy from /opt/rocm, so torch tensors and libfss share a device
+        # context.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        if not os.path.exists(LIB_PATH):
+            raise FssError(6, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(status):
+    if status != FSS_OK:
+        raise FssError(status, lib().fss_last_error().decode())
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _dev_ptr(t):
+    import torch
+    assert isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous(), "need a contiguous HIP tensor"
+    return C.c_void_p(t.data_ptr())
+
+
+# ---------------- grids and fields (host) ----------------
+
+def grid_icosahedral(level):
+    """Icosahedral vertices (icos_init order) and faces after `level` refinements."""
+    nv = 10 * 4 ** level + 2
+    nf = 20 * 4 ** level
+    x, y, z = (np.zeros(nv) for _ in range(3))
+    tris = np.zeros(3 * nf, dtype=np.int32)
+    _check(lib().fss_grid_icosahedral(level, _np_ptr(x), _np_ptr(y), _np_ptr(z), _np_ptr(tris)))
+    return x, y, z, tris.reshape(nf, 3)
+
+
+def voronoi_areas(x, y, z, tris):
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    a = np.zeros(len(x))
+    _check(lib().fss_grid_voronoi_areas(len(x), _np_ptr(x), _np_ptr(y), _np_ptr(z), _np_ptr(tris),
+                                        tris.shape[0], _np_ptr(a)))
+    return a
+
+
+def third_areas(x, y, z, tris):
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    a = np.zeros(len(x))
+    _check(lib().fss_grid_third_areas(len(x), _np_ptr(x), _np_ptr(y), _np_ptr(z), _np_ptr(tris),
+                                      tris.shape[0], _np_ptr(a)))
+    return a
+
+
+def grid_random(n, seed):
+    x, y, z = (np.zeros(n) for _ in range(3))
+    _check(lib().fss_grid_random(n, seed, _np_ptr(x), _np_ptr(y), _np_ptr(z)))
+    return x, y, z
+
+
+def rossby_haurwitz(x, y, z, w0=0.5, K=0.5):
+    v = np.zeros(len(x))
+    tr = np.zeros(len(x))
+    _check(lib().fss_field_rossby_haurwitz(len(x), _np_ptr(x), _np_ptr(y), _np_ptr(z), w0, K,
+                                           _np_ptr(v), _np_ptr(tr)))
+    return v, tr
+
+
+def gauss_vortices(x, y, z, area, nv=12, amp_std=2.0, rmin=0.05, rmax=0.3, seed=0):
+    v = np.zeros(len(x))
+    _check(lib().fss_field_gauss_vortices(len(x), _np_ptr(x), _np_ptr(y), _np_ptr(z), _np_ptr(area),
+                                          nv, amp_std, rmin, rmax, seed, _np_ptr(v)))
+    return v
+
+
+def sample_targets(n, m, seed):
+    idx = np.zeros(m, dtype=np.int64)
+    _check(lib().fss_sample_targets(n, m, seed, _np_ptr(idx)))
+    return idx
+
+
+# ---------------- device context ----------------
+
+class Context:
+    """One fss_ctx: configuration (run_information fields), device tree, workspace."""
+
+    def __init__(self, theta=0.7, degree=2, many_count=10, tree_levels=1, omega=2 * np.pi,
+                 precision=FP64, assign_mode=ASSIGN_PARITY, eval_mode=EVAL_PARITY,
+                 node_rule=NODES_REFERENCE, device=0, stream=None):
+        import torch
+        L = lib()
+        cfg = Config()
+        _check(L.fss_config_default(C.byref(cfg)))
+        cfg.theta, cfg.degree, cfg.many_count, cfg.tree_levels = theta, degree, many_count, tree_levels
+        cfg.omega, cfg.precision, cfg.assign_mode = omega, precision, assign_mode
+        cfg.eval_mode, cfg.node_rule, cfg.device = eval_mode, node_rule, device
+        self.cfg = cfg
+        self.device = torch.device("cuda", device)
+        h = C.c_void_p()
+        _check(L.fss_create(C.byref(h), C.byref(cfg)))
+        self.h = h
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        _check(L.fss_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+        self.n = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fss_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(lib().fss_sync(self.h))
+
+    def _particles(self, x, y, z, vort, area):
+        return Particles(_dev_ptr(x).value, _dev_ptr(y).value, _dev_ptr(z).value,
+                         _dev_ptr(vort).value, _dev_ptr(area).value, x.numel())
+
+    def _out(self, m):
+        import torch
+        o = [torch.empty(m, dtype=torch.float64, device=self.device) for _ in range(4)]
+        return o, Velocity(*(t.data_ptr() for t in o))
+
+    def tree_build(self, x, y, z):
+        _check(lib().fss_tree_build(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), x.numel()))
+        self.n = x.numel()
+
+    def stats(self):
+        s = TreeStats()
+        _check(lib().fss_tree_stats_get(self.h, C.byref(s)))
+        return s
+
+    def tree_export(self):
+        s = self.stats()
+        L, n, nn = s.tree_levels, s.n, s.n_nodes
+        T_all = 20 * (4 ** (L + 1) - 1) // 3
+        loc = np.zeros(L * n, dtype=np.int32)
+        found = np.zeros(L * n, dtype=np.uint8)
+        counts = np.zeros(nn, dtype=np.int32)
+        offsets = np.zeros(nn, dtype=np.int32)
+        _check(lib().fss_tree_export(self.h, None, None, _np_ptr(counts), None, None, None, None, None))
+        ids = np.zeros(max(1, int(counts.sum())), dtype=np.int32)
+        inter = np.zeros(max(1, 7 * s.n_interactions), dtype=np.int32)
+        tri_info = np.zeros(4 * T_all)
+        tri_xyz = np.zeros(9 * T_all)
+        _check(lib().fss_tree_export(self.h, _np_ptr(loc), _np_ptr(found), _np_ptr(counts),
+                                     _np_ptr(offsets), _np_ptr(ids), _np_ptr(inter),
+                                     _np_ptr(tri_info), _np_ptr(tri_xyz)))
+        return dict(loc=loc.reshape(L, n), found=found.reshape(L, n).astype(bool), counts=counts,
+                    offsets=offsets, ids=ids[:int(counts.sum())],
+                    inter=inter[:7 * s.n_interactions].reshape(-1, 7),
+                    tri_info=tri_info.reshape(T_all, 4), tri_xyz=tri_xyz.reshape(T_all, 3, 3))
+
+    def velocity_direct(self, x, y, z, vort, area):
+        o, ov = self._out(x.numel())
+        p = self._particles(x, y, z, vort, area)
+        _check(lib().fss_velocity_direct(self.h, C.byref(p), C.byref(ov)))
+        return tuple(o)
+
+    def velocity_tree(self, x, y, z, vort, area):
+        o, ov = self._out(x.numel())
+        p = self._particles(x, y, z, vort, area)
+        _check(lib().fss_velocity_tree(self.h, C.byref(p), C.byref(ov)))
+        return tuple(o)
+
+    def velocity_direct_targets(self, x, y, z, vort, area, idx):
+        import torch
+        idx = idx.to(device=self.device, dtype=torch.int64).contiguous()
+        o, ov = self._out(idx.numel())
+        p = self._particles(x, y, z, vort, area)
+        _check(lib().fss_velocity_direct_targets(self.h, C.byref(p), _dev_ptr(idx), idx.numel(),
+                                                 C.byref(ov)))
+        return tuple(o)
+
+    def rk4_step(self, x, y, z, vort, area, dt, use_tree=False):
+        _check(lib().fss_rk4_step(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), _dev_ptr(vort),
+                                  _dev_ptr(area), x.numel(), dt, int(use_tree)))
+
+    def rk4_run_graph(self, x, y, z, vort, area, dt, steps, use_tree=False):
+        _check(lib().fss_rk4_run_graph(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), _dev_ptr(vort),
+                                       _dev_ptr(area), x.numel(), dt, int(use_tree), steps))
+
+    def project_points(self, x, y, z, vort):
+        _check(lib().fss_project_points(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), _dev_ptr(vort),
+                                        x.numel()))
+
+    def last_timing(self):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().fss_last_timing(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value

@@ -1,0 +1,202 @@
+/*
+ * fss.h -- C ABI of the MI355X-native BVE velocity evaluation (libfss.so).
+ *
+ * Drop-in boundary for the reference's per-stage velocity path.  The
+ * reference has no FFI: its boundary is a set of header-only C++ functions
+ * over the `run_information` god-struct (Old-BVE-code/struct_list.h:50-102).
+ * Each entry point below names the reference function(s) it replaces.
+ *
+ * Conventions
+ *   - Every function returns fss_status; nothing throws across the ABI.
+ *     fss_last_error() gives a message for the calling thread's last failure.
+ *   - Particle arrays are SoA, FP64, and live in DEVICE memory (hipMalloc or
+ *     torch tensors), unless a function says "host".  The caller owns them.
+ *   - Outputs are fully overwritten (the reference zero-fills `modify`,
+ *     rhs_eval.h:48).
+ *   - Calls are asynchronous on the context's stream; fss_sync() waits.
+ *     A context must not be used from two threads at once.
+ *   - The context owns the device tree, the interaction lists and workspace.
+ */
+#ifndef FSS_H
+#define FSS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FSS_OK = 0,
+    FSS_ERR_INVALID = 1,   /* bad argument or configuration */
+    FSS_ERR_HIP = 2,       /* HIP runtime error (message via fss_last_error) */
+    FSS_ERR_NOMEM = 3,     /* device or host allocation failed */
+    FSS_ERR_STATE = 4,     /* call order (e.g. velocity_tree before tree_build) */
+    FSS_ERR_NUMERIC = 5,   /* singular interpolation matrix, duplicate particles */
+    FSS_ERR_NODEVICE = 6   /* no HIP device visible */
+} fss_status;
+
+typedef enum { FSS_FP64 = 0, FSS_MIXED = 1 } fss_precision;
+
+/* Point location (icos_utils.h:145-170).  PARITY keeps the reference rule:
+ * the first child whose three raw barycentric coordinates are >= 0 wins and a
+ * particle that passes no child is dropped from that level (its location
+ * stays 0).  ROBUST takes the child with the largest minimum coordinate when
+ * none passes, so no particle is ever dropped. */
+typedef enum { FSS_ASSIGN_PARITY = 0, FSS_ASSIGN_ROBUST = 1 } fss_assign_mode;
+
+/* Far-field evaluation.  PARITY reproduces the reference dispatch
+ * (rhs_eval.h:34-45: pp->pp, pc->pp, cp->cp, cc->cp) with planar proxies and
+ * raw barycentric coordinates (fast_rhs.h:173-236).  BLTC is the project's
+ * barycentric-Lagrange treecode: pp, pc (source proxies with modified
+ * charges), cp, cc, proxies on the sphere, normalised barycentric
+ * coordinates; judged against the direct sum. */
+typedef enum { FSS_EVAL_PARITY = 0, FSS_EVAL_BLTC = 1 } fss_eval_mode;
+
+/* Interpolation node rule (interp_utils.h:12-31).  REFERENCE runs fekete_init
+ * as written with its uninitialised `b` taken as 0 (at degree 2 this is the
+ * uniform 6-point lattice); SIN2 is the evident intended rule b = 1 - a - c
+ * with each coordinate mapped by sin^2(pi x / 2) and normalised. */
+typedef enum { FSS_NODES_REFERENCE = 0, FSS_NODES_SIN2 = 1 } fss_node_rule;
+
+/* Mirrors the run_information fields the velocity path reads
+ * (struct_list.h:58,77-80,88-90; new_dynamics.cpp:43,78-79). */
+typedef struct {
+    double theta;        /* MAC (r_t + r_s)/d < theta       struct_list.h:80 (default 0.7) */
+    int degree;          /* interpolation degree            new_dynamics.cpp:79 (default 2) */
+    int many_count;      /* cluster threshold               new_dynamics.cpp:78 (default 10) */
+    int tree_levels;     /* levels 0..tree_levels-1 used    struct_list.h:79 (default 1) */
+    double radius;       /* sphere radius                   struct_list.h:58 (1.0) */
+    double omega;        /* Coriolis Omega                  new_dynamics.cpp:43 (2 pi) */
+    int precision;       /* fss_precision */
+    int assign_mode;     /* fss_assign_mode */
+    int eval_mode;       /* fss_eval_mode */
+    int node_rule;       /* fss_node_rule */
+    int device;          /* HIP device ordinal */
+} fss_config;
+
+typedef struct fss_ctx fss_ctx;
+
+/* SoA particle state, device pointers, n particles. */
+typedef struct {
+    const double *x, *y, *z; /* positions on the unit sphere */
+    const double *vort;      /* relative vorticity omega_i (state slot 3, struct_list.h:65) */
+    const double *area;      /* area weight A_i (new_dynamics.cpp:57) */
+    int64_t n;
+} fss_particles;
+
+/* SoA output, device pointers: velocity (modify slots 0-2) and d(omega)/dt
+ * (slot 3), rhs_eval.h:54-55. */
+typedef struct {
+    double *u, *v, *w, *dvort;
+} fss_velocity;
+
+/* Tree statistics after fss_tree_build. */
+typedef struct {
+    int64_t n;                 /* particles */
+    int tree_levels;
+    int64_t n_nodes;           /* nodes on levels 0..tree_levels-1 */
+    int64_t n_interactions;    /* length of the interaction list */
+    int64_t n_type[4];         /* pp, pc, cp, cc counts */
+    int64_t dropped[32];       /* particles absent from each level (parity assignment) */
+    int64_t pairs_near;        /* sum n_t*n_s over interactions evaluated directly */
+    int64_t pairs_far;         /* kernel evaluations of the far field (proxy work) */
+    double mac_margin;         /* min |sep - theta| over every MAC with d > 0 */
+    int n_generations;         /* traversal generations (FIFO depth) */
+} fss_tree_stats;
+
+/* ---- context ---- */
+fss_status fss_config_default(fss_config *cfg);
+fss_status fss_create(fss_ctx **ctx, const fss_config *cfg);
+fss_status fss_destroy(fss_ctx *ctx);
+/* Use an external hipStream_t (e.g. torch's current stream); NULL = own stream. */
+fss_status fss_set_stream(fss_ctx *ctx, void *hip_stream);
+fss_status fss_sync(fss_ctx *ctx);
+const char *fss_last_error(void);
+const char *fss_status_string(fss_status s);
+int fss_version(void);
+
+/* ---- tree (replaces icosahedron_init icos_utils.h:135, interp_init
+ * interp_utils.h:61, points_assign icos_utils.h:172 and tree_traverse
+ * fast_rhs.h:16, called in that order by new_dynamics.cpp:84-88) ----
+ * x, y, z: device pointers.  Builds the icosahedral tree, locates the
+ * particles, forms the per-node particle lists and the FIFO interaction
+ * list, all on the device. */
+fss_status fss_tree_build(fss_ctx *ctx, const double *x, const double *y, const double *z,
+                          int64_t n);
+fss_status fss_tree_stats_get(fss_ctx *ctx, fss_tree_stats *out);
+
+/* Copies integer tree data to HOST buffers for bit-exact checks (any pointer
+ * may be NULL to skip it):
+ *   loc      [tree_levels * n]  node of each particle per level (tree_point_locs)
+ *   found    [tree_levels * n]  1 if the particle is on that level's node list
+ *   counts   [n_nodes]          particles per node, levels concatenated
+ *   offsets  [n_nodes]          start of each node list inside ids
+ *   ids      [sum of counts]    node lists (tree_tri_points), ascending ids
+ *   inter    [7 * n_interactions] {lev_t, lev_s, idx_t, idx_s, n_t, n_s, type}
+ *                               type 0 pp, 1 pc, 2 cp, 3 cc (interaction_pair)
+ *   tri_info [4 * n_tri_all]    centre xyz + radius, levels 0..tree_levels
+ *   tri_xyz  [9 * n_tri_all]    vertex coordinates of each triangle, in
+ *                               tree_tri_verts order */
+fss_status fss_tree_export(fss_ctx *ctx, int32_t *loc, uint8_t *found, int32_t *counts,
+                           int32_t *offsets, int32_t *ids, int32_t *inter, double *tri_info,
+                           double *tri_xyz);
+
+/* ---- per-stage velocity (replaces rhs_func rhs_eval.h:47-56) ----
+ * u = -1/(4 pi) sum_{j != i} (x_i x x_j) omega_j A_j / (1 - x_i . x_j),
+ * dvort = -2 Omega u_z.  Precision per fss_config.precision. */
+fss_status fss_velocity_direct(fss_ctx *ctx, const fss_particles *p, fss_velocity *out);
+fss_status fss_velocity_tree(fss_ctx *ctx, const fss_particles *p, fss_velocity *out);
+/* Direct sum at m target particles idx[0..m) (device int64 array); outputs
+ * have m entries.  FP64 regardless of the context precision: this is the
+ * accuracy reference on sampled targets. */
+fss_status fss_velocity_direct_targets(fss_ctx *ctx, const fss_particles *p, const int64_t *idx,
+                                       int64_t m, fss_velocity *out);
+
+/* ---- RK4 step with projection (new_dynamics.cpp:142-238 intent;
+ * project_points helpers.h:362-372) on device state x, y, z, vort (updated in
+ * place); use_tree selects the frozen t=0 tree (fss_tree_build first). */
+fss_status fss_rk4_step(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
+                        const double *area, int64_t n, double dt, int use_tree);
+/* Same, replayed `steps` times from a hipGraph captured on the first call. */
+fss_status fss_rk4_run_graph(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
+                             const double *area, int64_t n, double dt, int use_tree, int steps);
+/* project_points (helpers.h:362-372) on device state. */
+fss_status fss_project_points(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
+                              int64_t n);
+
+/* ---- timing of the last velocity call's kernels (HIP events on the
+ * context stream): total stage ms and the dominant kernel's ms. ---- */
+fss_status fss_last_timing(fss_ctx *ctx, double *stage_ms, double *main_kernel_ms);
+
+/* ---- grids and fields (host functions, host buffers) ----
+ * Icosahedral particle grid after `level` refinements: V = 10*4^level + 2
+ * vertices in icos_init creation order (icos_utils.h:15-133), F = 20*4^level
+ * faces (tree_tri_verts of that level).  Replaces the missing
+ * {N}points_rh4.csv / {N}tris.csv inputs (helpers.h:299-342). */
+fss_status fss_grid_icosahedral(int level, double *x, double *y, double *z, int32_t *tris);
+/* Dual-Voronoi cell areas of a triangulated sphere (sum 4 pi). */
+fss_status fss_grid_voronoi_areas(int64_t n, const double *x, const double *y, const double *z,
+                                  const int32_t *tris, int64_t ntri, double *area);
+/* area_init rule (helpers.h:344-360): a third of each adjacent triangle. */
+fss_status fss_grid_third_areas(int64_t n, const double *x, const double *y, const double *z,
+                                const int32_t *tris, int64_t ntri, double *area);
+/* Uniform random points on the sphere (normalised Gaussian triples). */
+fss_status fss_grid_random(int64_t n, uint64_t seed, double *x, double *y, double *z);
+/* Rossby-Haurwitz vorticity 2 w0 sin(lat) - 30 K sin(lat) cos^4(lat) cos(4 lon)
+ * (vorticity_functions.h:13-24 family); tracer (optional) = 2 + lat. */
+fss_status fss_field_rossby_haurwitz(int64_t n, const double *x, const double *y, const double *z,
+                                     double w0, double K, double *vort, double *tracer);
+/* Sum of nv Gaussian vortices A_k exp(-|x - c_k|^2 / r_k^2) (chordal), centres
+ * uniform on the sphere, A_k ~ N(0, amp_std^2), r_k ~ U(rmin, rmax), minus the
+ * area-weighted mean (vorticity_functions.h:26-46 family). */
+fss_status fss_field_gauss_vortices(int64_t n, const double *x, const double *y, const double *z,
+                                    const double *area, int nv, double amp_std, double rmin,
+                                    double rmax, uint64_t seed, double *vort);
+/* m distinct indices of [0, n) (partial Fisher-Yates) for error sampling. */
+fss_status fss_sample_targets(int64_t n, int64_t m, uint64_t seed, int64_t *idx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSS_H */

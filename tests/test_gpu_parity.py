@@ -1,0 +1,165 @@
+"""GPU parity: libfss (HIP, through the C ABI) against the CPU oracle on the
+same seeded inputs.  Bars: integer tree/list data bit-exact, FP64 velocities
+rel-L2 <= 1e-12, mixed precision rel-L2 <= 1e-6 against FP64."""
+import numpy as np
+import pytest
+
+from conftest import icos_case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12
+TOL_MIXED = 1e-6
+
+
+def _dev(*arrs):
+    import torch
+    return [torch.from_numpy(np.ascontiguousarray(a)).to("cuda") for a in arrs]
+
+
+def _host(*ts):
+    return [t.cpu().numpy() for t in ts]
+
+
+def _vel_err(got, ref):
+    U = np.stack(got[:3], 1)
+    R = np.stack(ref[:3], 1)
+    return rel_l2(U, R), rel_l2(got[3], ref[3])
+
+
+@pytest.fixture(scope="module")
+def case4(fsslib):
+    return icos_case(fsslib, 4)
+
+
+@pytest.fixture(scope="module")
+def case5(fsslib):
+    return icos_case(fsslib, 5)
+
+
+def test_direct_fp64_matches_oracle(fsslib, orc, case4):
+    x, y, z, vort, area, _ = case4
+    print(f"direct fp64 N={len(x)}", flush=True)
+    ctx = fsslib.Context(precision=fsslib.FP64)
+    d = _dev(x, y, z, vort, area)
+    got = _host(*ctx.velocity_direct(*d))
+    ref = orc.rhs_direct(x, y, z, vort, area)
+    eu, ew = _vel_err(got, ref)
+    print("rel-L2 u", eu, "dvort", ew)
+    assert eu <= TOL_FP64 and ew <= TOL_FP64
+
+
+def test_direct_mixed_within_1e6(fsslib, orc, case5):
+    x, y, z, vort, area, _ = case5
+    print(f"direct mixed N={len(x)}", flush=True)
+    ctx = fsslib.Context(precision=fsslib.MIXED)
+    d = _dev(x, y, z, vort, area)
+    got = _host(*ctx.velocity_direct(*d))
+    ref = orc.rhs_direct(x, y, z, vort, area)
+    eu, ew = _vel_err(got, ref)
+    print("rel-L2 u", eu, "dvort", ew)
+    assert eu <= TOL_MIXED and ew <= TOL_MIXED
+
+
+def test_direct_targets_subset(fsslib, orc, case5):
+    import torch
+    x, y, z, vort, area, _ = case5
+    idx = fsslib.sample_targets(len(x), 777, 11)
+    ctx = fsslib.Context(precision=fsslib.MIXED)  # targets entry point is FP64 regardless
+    d = _dev(x, y, z, vort, area)
+    got = _host(*ctx.velocity_direct_targets(*d, torch.from_numpy(idx)))
+    ref = orc.rhs_direct(x, y, z, vort, area, idx=idx)
+    eu, ew = _vel_err(got, ref)
+    assert eu <= TOL_FP64 and ew <= TOL_FP64
+
+
+@pytest.mark.parametrize("L,assign", [(4, 0), (6, 0), (4, 1), (6, 1)])
+def test_tree_integer_data_bit_exact(fsslib, orc, case5, L, assign):
+    x, y, z, vort, area, _ = case5
+    print(f"tree build N={len(x)} L={L} assign={assign}", flush=True)
+    ctx = fsslib.Context(tree_levels=L, assign_mode=assign)
+    d = _dev(x, y, z)
+    ctx.tree_build(*d)
+    g = ctx.tree_export()
+    T = orc.Tree(x, y, z, L, theta=0.7, many_count=10, degree=2, assign=assign)
+    # geometry (levels 0..L)
+    np.testing.assert_array_equal(g["tri_info"], T.tri_info())
+    vx = T.verts()[T.tri_verts()]
+    np.testing.assert_array_equal(g["tri_xyz"], vx)
+    # point location
+    np.testing.assert_array_equal(g["found"], T.found())
+    np.testing.assert_array_equal(g["loc"], T.loc())
+    np.testing.assert_array_equal(g["counts"], T.counts())
+    np.testing.assert_array_equal(g["ids"], T.ids())
+    # interaction list, in reference (FIFO) order
+    inter = T.interactions()
+    print("interactions", inter.shape[0], "mac margin", T.mac_margin)
+    assert g["inter"].shape == inter.shape
+    np.testing.assert_array_equal(g["inter"], inter)
+    s = ctx.stats()
+    assert s.n_interactions == inter.shape[0]
+
+
+@pytest.mark.parametrize("L,assign,evalm,rule", [(4, 0, 0, 0), (6, 0, 0, 0), (6, 1, 0, 0),
+                                                 (5, 1, 1, 1)])
+def test_tree_velocity_fp64_matches_oracle(fsslib, orc, case5, L, assign, evalm, rule):
+    x, y, z, vort, area, _ = case5
+    print(f"tree fp64 N={len(x)} L={L} assign={assign} eval={evalm}", flush=True)
+    ctx = fsslib.Context(tree_levels=L, assign_mode=assign, eval_mode=evalm, node_rule=rule)
+    d = _dev(x, y, z, vort, area)
+    ctx.tree_build(*d[:3])
+    got = _host(*ctx.velocity_tree(*d))
+    T = orc.Tree(x, y, z, L, theta=0.7, many_count=10, degree=2, assign=assign, eval_mode=evalm,
+                 node_rule=rule)
+    ref = T.rhs(vort, area, nthreads=8)
+    eu, ew = _vel_err(got, ref)
+    dref = orc.rhs_direct(x, y, z, vort, area)
+    print("rel-L2 vs oracle tree", eu, ew, " tree vs direct", _vel_err(ref, dref)[0])
+    assert eu <= TOL_FP64 and ew <= TOL_FP64
+
+
+@pytest.mark.parametrize("evalm,rule", [(0, 0), (1, 1)])
+def test_tree_velocity_mixed_within_1e6(fsslib, case5, evalm, rule):
+    x, y, z, vort, area, _ = case5
+    d = _dev(x, y, z, vort, area)
+    c64 = fsslib.Context(tree_levels=5, assign_mode=1, eval_mode=evalm, node_rule=rule)
+    c32 = fsslib.Context(tree_levels=5, assign_mode=1, eval_mode=evalm, node_rule=rule,
+                         precision=fsslib.MIXED)
+    c64.tree_build(*d[:3])
+    c32.tree_build(*d[:3])
+    a = _host(*c64.velocity_tree(*d))
+    b = _host(*c32.velocity_tree(*d))
+    eu, ew = _vel_err(b, a)
+    print("mixed vs fp64 tree", eu, ew)
+    assert eu <= TOL_MIXED and ew <= TOL_MIXED
+
+
+@pytest.mark.parametrize("use_tree", [False, True])
+def test_rk4_step_matches_oracle(fsslib, orc, case4, use_tree):
+    x, y, z, vort, area, _ = case4
+    dt = 0.01
+    ctx = fsslib.Context(tree_levels=4, assign_mode=1)
+    d = _dev(x, y, z, vort, area)
+    T = None
+    if use_tree:
+        ctx.tree_build(*d[:3])
+        T = orc.Tree(x, y, z, 4, assign=1)
+    ctx.rk4_step(d[0], d[1], d[2], d[3], d[4], dt, use_tree=use_tree)
+    got = _host(*d[:4])
+    ref = orc.rk4_step(x, y, z, vort, area, dt, tree=T)
+    for g, r in zip(got, ref):
+        assert rel_l2(g, r) <= TOL_FP64
+
+
+def test_rk4_graph_replays_match_eager(fsslib, case4):
+    x, y, z, vort, area, _ = case4
+    dt = 0.01
+    ctx = fsslib.Context(tree_levels=4, assign_mode=1)
+    a = _dev(x, y, z, vort, area)
+    b = _dev(x, y, z, vort, area)
+    ctx.tree_build(*a[:3])
+    for _ in range(3):
+        ctx.rk4_step(a[0], a[1], a[2], a[3], a[4], dt, use_tree=True)
+    ctx.rk4_run_graph(b[0], b[1], b[2], b[3], b[4], dt, 3, use_tree=True)
+    for g, r in zip(_host(*b[:4]), _host(*a[:4])):
+        np.testing.assert_array_equal(g, r)
