@@ -261,12 +261,12 @@ fss_status fss_create(fss_ctx **out, const fss_config *cfg)
             return set_error("hipStreamCreate failed"), FSS_ERR_HIP;
         }
         c->own_stream = true;
-        for (auto &e : c->timing.ev) {
-            if (hipEventCreate(&e) != hipSuccess) {
-                fss_destroy(c);
-                return set_error("hipEventCreate failed"), FSS_ERR_HIP;
-            }
-        }
+        for (auto &row : c->timing.ev)
+            for (auto &e : row)
+                if (hipEventCreate(&e) != hipSuccess) {
+                    fss_destroy(c);
+                    return set_error("hipEventCreate failed"), FSS_ERR_HIP;
+                }
         fss_status s = interp_setup(c);
         if (s != FSS_OK) {
             fss_destroy(c);
@@ -284,8 +284,9 @@ fss_status fss_destroy(fss_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    for (auto &e : ctx->timing.ev)
-        if (e) (void)hipEventDestroy(e);
+    for (auto &row : ctx->timing.ev)
+        for (auto &e : row)
+            if (e) (void)hipEventDestroy(e);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx; // DevBuf members free their device memory
     return FSS_OK;
@@ -403,14 +404,46 @@ fss_status fss_project_points(fss_ctx *ctx, double *x, double *y, double *z, dou
 fss_status fss_last_timing(fss_ctx *ctx, double *stage_ms, double *main_ms)
 {
     if (!ctx) return set_error("null context"), FSS_ERR_INVALID;
-    if (!ctx->timing.valid) return set_error("no timed call yet"), FSS_ERR_STATE;
-    hipEvent_t *ev = ctx->timing.ev;
+    if (!ctx->timing.valid || ctx->timing.cur < 0) return set_error("no timed call yet"), FSS_ERR_STATE;
+    hipEvent_t *ev = ctx->timing.slot();
     FSS_HIP(hipEventSynchronize(ev[3]));
     float a = 0, b = 0;
     FSS_HIP(hipEventElapsedTime(&a, ev[0], ev[3]));
     FSS_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
     if (stage_ms) *stage_ms = a;
     if (main_ms) *main_ms = b;
+    return FSS_OK;
+}
+
+fss_status fss_timing_reset(fss_ctx *ctx)
+{
+    if (!ctx) return set_error("null context"), FSS_ERR_INVALID;
+    ctx->timing.calls = 0;
+    ctx->timing.cur = -1;
+    ctx->timing.valid = false;
+    return FSS_OK;
+}
+
+fss_status fss_timing_mean(fss_ctx *ctx, int64_t *calls, double *stage_ms, double *main_ms)
+{
+    if (!ctx) return set_error("null context"), FSS_ERR_INVALID;
+    Timing &t = ctx->timing;
+    if (!t.valid || t.calls < 1) return set_error("no timed call yet"), FSS_ERR_STATE;
+    if (t.calls > Timing::RING)
+        return set_error("more timed calls than event slots; reset more often"), FSS_ERR_STATE;
+    double sa = 0, sb = 0;
+    for (int64_t i = 0; i < t.calls; i++) {
+        hipEvent_t *ev = t.ev[i];
+        FSS_HIP(hipEventSynchronize(ev[3]));
+        float a = 0, b = 0;
+        FSS_HIP(hipEventElapsedTime(&a, ev[0], ev[3]));
+        FSS_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+        sa += a;
+        sb += b;
+    }
+    if (calls) *calls = t.calls;
+    if (stage_ms) *stage_ms = sa / t.calls;
+    if (main_ms) *main_ms = sb / t.calls;
     return FSS_OK;
 }
 

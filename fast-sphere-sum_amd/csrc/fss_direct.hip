@@ -7,8 +7,10 @@
 // j == i is skipped as the reference does (rhs_eval.h:20); only the tile that
 // holds the block's own targets pays for the index test.
 //
-// FP64: reference kernel (x_i x x_j) w_j / (1 - x_i . x_j), factored as
-//       x_i x sum_j (w_j / (1 - x_i.x_j)) x_j.
+// FP64: reference kernel (x_i x x_j) w_j / (1 - x_i . x_j) with the
+//       reference's unfused x.y, factored as
+//       x_i x sum_j (w_j / (1 - x_i.x_j)) (x_j - x_i)  (no cancellation for
+//       near pairs).
 // FP32 (mixed): 1 - x.y = |x - y|^2 / 2 on the sphere, d = x_j - x_i,
 //       x_i x x_j = x_i x d; per-tile FP32 partial sums are added into FP64
 //       accumulators (256 terms per partial), the cross product and scaling
@@ -54,12 +56,20 @@ __device__ inline void tile64(const double4 *tile, int cnt, int64_t base, int64_
 #pragma unroll 8
     for (int k = 0; k < cnt; k++) {
         double4 s = tile[k];
-        double den = 1.0 - (xt.x * s.x + xt.y * s.y + xt.z * s.z);
+        double dot;
+        {
+            // x.y summed as the reference does (helpers.h:23-28): no FMA
+#pragma clang fp contract(off)
+            dot = xt.x * s.x;
+            dot = dot + xt.y * s.y;
+            dot = dot + xt.z * s.z;
+        }
+        double den = 1.0 - dot;
         double c = s.w * rcp64d(den);
         if (MASK) c = (base + k == self) ? 0.0 : c;
-        S0 = fma(c, s.x, S0);
-        S1 = fma(c, s.y, S1);
-        S2 = fma(c, s.z, S2);
+        S0 = fma(c, s.x - xt.x, S0);
+        S1 = fma(c, s.y - xt.y, S1);
+        S2 = fma(c, s.z - xt.z, S2);
     }
 }
 
@@ -165,7 +175,8 @@ fss_status velocity_direct(fss_ctx *ctx, const fss_particles *p, const int64_t *
     if (!idx) m = n;
     const double scale = -1.0 / (4.0 * M_PI);
     const bool mixed = !force_fp64 && ctx->cfg.precision == FSS_MIXED;
-    hipEvent_t *ev = ctx->timing.ev;
+    if (!ctx->capturing) timing_begin(ctx->timing);
+    hipEvent_t *ev = ctx->capturing ? nullptr : ctx->timing.slot();
     if (!ctx->capturing) FSS_HIP(hipEventRecord(ev[0], s));
     unsigned gp = (unsigned)div_up(n, 256), gt = (unsigned)div_up(m, DTILE);
     if (!mixed) {

@@ -3,7 +3,8 @@
 // Reference map (Old-BVE-code/):
 //   rhs_fast_sum  rhs_eval.h:34-45   loop over the interaction list
 //   pp            fast_rhs.h:93-118  -> k_near (class A, particle sources)
-//   cp            fast_rhs.h:173-236 -> k_proxy (class B) + interpolation in k_near
+//   cp            fast_rhs.h:173-236 -> k_proxy + k_solve (class B) and the
+//                                       interpolation in k_combine
 //   rhs_func      rhs_eval.h:47-56   -> k_combine (sum over levels, -1/(4 pi), dw/dt)
 //
 // Layout in HBM (built by fss_tree.hip):
@@ -13,21 +14,24 @@
 //                         the modified charges of the upward pass in BLTC)
 //   ranges                (start, count) source ranges, grouped by target node
 //                         (class A: direct targets, class B: proxy targets)
-// Because interpolation is linear, the proxy values of all class-B
-// interactions of one target node are summed first (k_proxy) and interpolated
-// once (in k_near); the reference interpolates per interaction.
+// Interpolation is linear, so the proxy values of all class-B interactions of
+// a target node are summed first (k_proxy, k_solve) and interpolated once per
+// particle (k_combine); the reference interpolates per interaction.
 //
-// Work decomposition: one wave64 per work item; lanes are (target t, phase
-// ph) with T = pow2 >= #targets and P = 64/T phases; phase ph walks the
-// sources ph, ph+P, ...; the P partial sums of a target are combined with
-// cross-lane shuffles.  Per-target results go to a per-position buffer, and
-// k_combine sums the levels of each particle in a fixed order (deterministic,
-// no atomics).
+// Near and proxy kernels: one wave64 per work item.  Sources are staged in
+// LDS chunks of NCH records; lanes are (target group, phase): each lane owns
+// R targets (register blocking: every LDS read feeds R pair evaluations) and
+// phase ph walks the chunk entries ph, ph+P, ...; the phase partials are
+// summed through LDS in a fixed order (deterministic, no atomics).  Ranges
+// whose source node contains the target node are evaluated in a second pass
+// that skips the self pair by particle id, as the reference does.
 #include <cmath>
 
 #include "fss_internal.h"
 
 namespace fss {
+
+constexpr int NCH = 256;
 
 // ------------------------------------------------------------------
 // shared device helpers
@@ -45,25 +49,90 @@ __device__ inline double rcp64(double d)
 
 __device__ inline float rcp32(float d) { return __builtin_amdgcn_rcpf(d); }
 
-__device__ inline double shfl_xor_d(double v, int mask)
+template <typename Real> struct Rec;
+template <> struct Rec<double> {
+    using T4 = double4;
+};
+template <> struct Rec<float> {
+    using T4 = float4;
+};
+
+// x.y as the reference sums it (helpers.h:23-28): products rounded
+// separately, left to right, no FMA -- near pairs make 1 - x.y ill
+// conditioned, so its rounding must match the reference's
+__device__ inline double dot_ref(const double4 &a, const double4 &b)
 {
-    int2 a = *reinterpret_cast<int2 *>(&v);
-    a.x = __shfl_xor(a.x, mask, 64);
-    a.y = __shfl_xor(a.y, mask, 64);
-    return *reinterpret_cast<double *>(&a);
+#pragma clang fp contract(off)
+    double s = a.x * b.x;
+    s = s + a.y * b.y;
+    s = s + a.z * b.z;
+    return s;
 }
 
-__device__ inline float shfl_xor_f(float v, int mask) { return __shfl_xor(v, mask, 64); }
-
-__device__ inline int pow2_at_least(int c)
+// One pair evaluation.  The three components accumulate c * (x_j - x_i): the
+// caller takes x_i x S at the end (x_i x x_j = x_i x (x_j - x_i)), which keeps
+// near pairs free of cancellation.
+// FP64 near field: reference denominator 1 - x.y.
+__device__ inline void pair_eval(const double4 &xt, const double4 &s, double, double &S0, double &S1,
+                                 double &S2)
 {
-    int t = 1;
-    while (t < c) t <<= 1;
-    return t;
+    double den = 1.0 - dot_ref(xt, s);
+    double c = s.w * rcp64(den);
+    S0 = fma(c, s.x - xt.x, S0);
+    S1 = fma(c, s.y - xt.y, S1);
+    S2 = fma(c, s.z - xt.z, S2);
 }
 
-// 3x3 solve in reference order (duplicated from fss_geom.h without the
-// host/device contraction requirement: interpolation is not bit-exact work)
+// FP32: records are offsets from a common node centre; 1 - x.y = |d|^2/2 +
+// off, where off = (1 - |y|^2)/2 for a target off the sphere (planar proxy).
+__device__ inline void pair_eval(const float4 &xt, const float4 &s, float off, float &S0, float &S1,
+                                 float &S2)
+{
+    float dx = s.x - xt.x, dy = s.y - xt.y, dz = s.z - xt.z;
+    float den = fmaf(0.5f, dx * dx + dy * dy + dz * dz, off);
+    float c = s.w * rcp32(den);
+    S0 = fmaf(c, dx, S0);
+    S1 = fmaf(c, dy, S1);
+    S2 = fmaf(c, dz, S2);
+}
+
+// FP64 far field (well separated: no cancellation): y_k x sum c_j x_j
+__device__ inline void pair_far(const double4 &xt, const double4 &s, double, double &S0, double &S1,
+                                double &S2)
+{
+    double den = 1.0 - (xt.x * s.x + xt.y * s.y + xt.z * s.z);
+    double c = s.w * rcp64(den);
+    S0 = fma(c, s.x, S0);
+    S1 = fma(c, s.y, S1);
+    S2 = fma(c, s.z, S2);
+}
+
+__device__ inline void pair_far(const float4 &xt, const float4 &s, float off, float &S0, float &S1,
+                                float &S2)
+{
+    pair_eval(xt, s, off, S0, S1, S2);
+}
+
+// Shift that brings a range's records to the target node's frame: zero for
+// FP64 (absolute coordinates), centre(range node) - centre(target node) for
+// the FP32 offset records.
+__device__ inline double4 range_shift(const double4 *, const double *, int, int)
+{
+    return make_double4(0, 0, 0, 0);
+}
+__device__ inline float4 range_shift(const float4 *, const double *__restrict__ centres, int src_node,
+                                     int tgt_node)
+{
+    const double *a = centres + 4 * (int64_t)src_node, *b = centres + 4 * (int64_t)tgt_node;
+    return make_float4((float)(a[0] - b[0]), (float)(a[1] - b[1]), (float)(a[2] - b[2]), 0.0f);
+}
+__device__ inline double4 shifted(const double4 &s, const double4 &) { return s; }
+__device__ inline float4 shifted(const float4 &s, const float4 &d)
+{
+    return make_float4(s.x + d.x, s.y + d.y, s.z + d.z, s.w);
+}
+
+// 3x3 solve in reference order (interpolation coordinates; not bit-exact work)
 __device__ inline void bary_solve(const double *T, const double *p, double *c)
 {
     double a[9] = {T[0], T[1], T[2], T[3], T[4], T[5], T[6], T[7], T[8]};
@@ -112,18 +181,23 @@ __device__ inline void bary_solve(const double *T, const double *p, double *c)
 }
 
 // monomials s^(i-j) t^j, index i(i+1)/2 + j (interp_utils.h:49-59)
-__device__ inline void monomials(double s, double t, int degree, double *m)
+template <int D> __device__ inline void monomials(double s, double t, double *m)
 {
-    double sp[16], tp[16];
+    double sp[D + 1], tp[D + 1];
     sp[0] = 1.0;
     tp[0] = 1.0;
-    for (int k = 1; k <= degree; k++) {
+#pragma unroll
+    for (int k = 1; k <= D; k++) {
         sp[k] = sp[k - 1] * s;
         tp[k] = tp[k - 1] * t;
     }
-    for (int i = 0; i <= degree; i++)
+#pragma unroll
+    for (int i = 0; i <= D; i++)
+#pragma unroll
         for (int j = 0; j <= i; j++) m[i * (i + 1) / 2 + j] = sp[i - j] * tp[j];
 }
+
+template <int D> constexpr int npts() { return (D + 1) * (D + 2) / 2; }
 
 // ------------------------------------------------------------------
 // gather: level-ordered sources
@@ -147,335 +221,398 @@ __global__ void k_gather64(const double *__restrict__ x, const double *__restric
 __global__ void k_gather32(const double *__restrict__ x, const double *__restrict__ y,
                            const double *__restrict__ z, const double *__restrict__ vort,
                            const double *__restrict__ area, const int32_t *__restrict__ ids,
-                           int64_t p0, const double4 *__restrict__ proxy, int64_t nprox,
+                           int64_t p0, const double4 *__restrict__ proxy, int64_t nprox, int p,
+                           const int32_t *__restrict__ posnode, const double *__restrict__ centres,
                            int planar, float4 *src, float *aoff)
 {
+    // FP32 records hold the offset from the centre of the record's node
+    // (FP64 subtraction, then rounding): differences of nearby points keep
+    // ~1e-9 absolute accuracy instead of the 6e-8 of rounded coordinates.
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < p0) {
         int32_t id = ids[i];
-        src[i] = make_float4((float)x[id], (float)y[id], (float)z[id], (float)(vort[id] * area[id]));
+        const double *c = centres + 4 * (int64_t)posnode[i];
+        src[i] = make_float4((float)(x[id] - c[0]), (float)(y[id] - c[1]), (float)(z[id] - c[2]),
+                             (float)(vort[id] * area[id]));
     } else if (i < p0 + nprox) {
         double4 q = proxy[i - p0];
-        float qx = (float)q.x, qy = (float)q.y, qz = (float)q.z;
-        src[i] = make_float4(qx, qy, qz, 0.0f);
+        const double *c = centres + 4 * ((i - p0) / p);
+        src[i] = make_float4((float)(q.x - c[0]), (float)(q.y - c[1]), (float)(q.z - c[2]), 0.0f);
         // For a particle x on the sphere, 1 - y.x = |x - y|^2/2 + (1 - |y|^2)/2.
-        // Planar proxies (PARITY) are off the sphere and carry that offset,
-        // computed from the rounded coordinates so the identity holds for them.
-        double nq = (double)qx * qx + (double)qy * qy + (double)qz * qz;
+        // Planar proxies (PARITY) are off the sphere and carry that offset.
+        double nq = q.x * q.x + q.y * q.y + q.z * q.z;
         aoff[i - p0] = planar ? (float)(0.5 * (1.0 - nq)) : 0.0f;
     }
 }
 
 // ------------------------------------------------------------------
 // upward pass (BLTC): modified charges q_k = sum_j L_k(b_j) w_j
+// one 256-thread block per charge node
 // ------------------------------------------------------------------
-template <typename Real>
-__global__ void k_upward(const int32_t *__restrict__ charge_nodes, int64_t ncharge,
-                         const int32_t *__restrict__ offsets, const int32_t *__restrict__ counts,
-                         const double *__restrict__ tri_xyz, const double *__restrict__ x,
-                         const double *__restrict__ y, const double *__restrict__ z,
-                         const double *__restrict__ vort, const double *__restrict__ area,
-                         const int32_t *__restrict__ ids, const double *__restrict__ vinv,
-                         int degree, int p, int64_t p0, Real *src4)
+template <typename Real, int D>
+__global__ void __launch_bounds__(256) k_upward(const int32_t *__restrict__ charge_nodes,
+                                                const int32_t *__restrict__ offsets,
+                                                const int32_t *__restrict__ counts,
+                                                const double *__restrict__ tri_xyz,
+                                                const double *__restrict__ x,
+                                                const double *__restrict__ y,
+                                                const double *__restrict__ z,
+                                                const double *__restrict__ vort,
+                                                const double *__restrict__ area,
+                                                const int32_t *__restrict__ ids,
+                                                const double *__restrict__ vinv, int64_t p0,
+                                                Real *src4)
 {
-    int64_t item = blockIdx.x;
-    if (item >= ncharge) return;
-    int lane = threadIdx.x;
-    int32_t g = charge_nodes[item];
+    constexpr int P = npts<D>();
+    __shared__ double red[4][P];
+    int32_t g = charge_nodes[blockIdx.x];
     int32_t off = offsets[g], cnt = counts[g];
     const double *T = tri_xyz + 9 * (int64_t)g;
-    double M[FSS_MAX_P];
-    for (int k = 0; k < p; k++) M[k] = 0.0;
-    for (int32_t j = lane; j < cnt; j += 64) {
+    double M[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) M[k] = 0.0;
+    for (int32_t j = threadIdx.x; j < cnt; j += 256) {
         int32_t id = ids[off + j];
-        double pt[3] = {x[id], y[id], z[id]}, c[3], m[FSS_MAX_P];
+        double pt[3] = {x[id], y[id], z[id]}, c[3], m[P];
         bary_solve(T, pt, c);
         double sg = c[0] + c[1] + c[2];
-        monomials(c[0] / sg, c[1] / sg, degree, m);
+        monomials<D>(c[0] / sg, c[1] / sg, m);
         double w = vort[id] * area[id];
-        for (int k = 0; k < p; k++) M[k] += m[k] * w;
+#pragma unroll
+        for (int k = 0; k < P; k++) M[k] = fma(m[k], w, M[k]);
     }
-    for (int k = 0; k < p; k++)
-        for (int o = 1; o < 64; o <<= 1) M[k] += shfl_xor_d(M[k], o);
-    // q_k = sum_idx Vinv[idx][k] M_idx   (vinv column-major: vinv[idx + k*p])
-    if (lane < p) {
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+        double v = M[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[wv][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < P) {
+        int k = threadIdx.x;
         double q = 0.0;
-        for (int idx = 0; idx < p; idx++) q += vinv[idx + lane * p] * M[idx];
-        int64_t pos = p0 + (int64_t)g * p + lane;
-        src4[4 * pos + 3] = (Real)q;  // weight slot of the (x, y, z, w) source record
+        for (int idx = 0; idx < P; idx++) {
+            double Mi = red[0][idx] + red[1][idx] + red[2][idx] + red[3][idx];
+            q = fma(vinv[idx + k * P], Mi, q); // vinv column-major: Vinv[idx][k]
+        }
+        int64_t pos = p0 + (int64_t)g * P + k;
+        src4[4 * pos + 3] = (Real)q; // weight slot of the (x, y, z, w) record
     }
 }
 
 // ------------------------------------------------------------------
-// proxy potentials (class B) and their interpolants
+// staged source loop shared by the near and proxy kernels
 // ------------------------------------------------------------------
-__device__ inline void lu_solve_small(int p, const double *lu, const int32_t *ipiv, double *b)
+// Stages the sources of ranges [r0, r1) that are not flagged self into LDS in
+// chunks and calls body(k) for every staged entry k owned by this lane.
+// Records are moved to the target node's frame (range_shift) while staged.
+template <typename T4, typename Body>
+__device__ inline void staged_ranges(const Range *__restrict__ ranges, int r0, int r1,
+                                     const T4 *__restrict__ src, const double *__restrict__ centres,
+                                     int tgt_node, T4 *lds, int ph, int P, bool act, Body body,
+                                     int &fill)
 {
-    for (int k = 0; k < p; k++) {
-        int q = ipiv[k];
-        if (q != k) {
-            double t = b[k];
-            b[k] = b[q];
-            b[q] = t;
+    const int lane = threadIdx.x;
+    for (int r = r0; r < r1; r++) {
+        Range rg = ranges[r];
+        if (rg.count & FSS_SELF_BIT) continue;
+        T4 sh = range_shift(src, centres, rg.node, tgt_node);
+        for (int base = 0; base < rg.count; base += 64) {
+            int m = min(64, rg.count - base);
+            if (fill + m > NCH) {
+                __syncthreads();
+                if (act)
+                    for (int k = ph; k < fill; k += P) body(k);
+                body(-1); // chunk boundary
+                __syncthreads();
+                fill = 0;
+            }
+            if (lane < m) lds[fill + lane] = shifted(src[rg.start + base + lane], sh);
+            fill += m;
         }
     }
-    for (int k = 0; k < p; k++)
-        if (b[k] != 0.0)
-            for (int i = k + 1; i < p; i++) b[i] = b[i] - b[k] * lu[i + k * p];
-    for (int k = p - 1; k >= 0; k--)
-        if (b[k] != 0.0) {
-            b[k] = b[k] / lu[k + k * p];
-            for (int i = 0; i < k; i++) b[i] = b[i] - b[k] * lu[i + k * p];
-        }
 }
 
-// FP64: reference kernel form (x cross y) / (1 - x.y), factored as
-// y_k x sum_j (w_j / (1 - y_k . x_j)) x_j.
-__global__ void __launch_bounds__(64) k_proxy64(const ProxyItem *__restrict__ items,
-                                                const int2 *__restrict__ ranges,
-                                                const double4 *__restrict__ src, int64_t p0, int p,
-                                                const double *__restrict__ lu,
-                                                const int32_t *__restrict__ ipiv, double *alpha)
+// target position as a double (absolute for FP64 records, from the input
+// arrays for FP32 offset records)
+__device__ inline void abs_pos(const double4 &r, int32_t, const double *, const double *,
+                               const double *, double *o)
 {
-    __shared__ double phi[3][64];
-    ProxyItem it = items[blockIdx.x];
-    int lane = threadIdx.x;
-    int T = pow2_at_least(p), P = 64 / T;
-    int k = lane & (T - 1), ph = lane / T;
-    int kk = k < p ? k : 0;
-    double4 yk = src[p0 + (int64_t)it.node * p + kk];
-    double S0 = 0, S1 = 0, S2 = 0;
-    for (int r = it.b_beg; r < it.b_end; r++) {
-        int2 rg = ranges[r];
-        for (int j = rg.x + ph; j < rg.x + rg.y; j += P) {
-            double4 s = src[j];
-            double den = 1.0 - (yk.x * s.x + yk.y * s.y + yk.z * s.z);
-            double c = s.w * rcp64(den);
-            S0 = fma(c, s.x, S0);
-            S1 = fma(c, s.y, S1);
-            S2 = fma(c, s.z, S2);
+    o[0] = r.x;
+    o[1] = r.y;
+    o[2] = r.z;
+}
+__device__ inline void abs_pos(const float4 &, int32_t id, const double *__restrict__ x,
+                               const double *__restrict__ y, const double *__restrict__ z, double *o)
+{
+    o[0] = x[id];
+    o[1] = y[id];
+    o[2] = z[id];
+}
+
+// ------------------------------------------------------------------
+// near field (class A): direct pairs, R targets per lane
+// ------------------------------------------------------------------
+template <typename Real, int R>
+__global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
+                                             const Range *__restrict__ ranges,
+                                             const typename Rec<Real>::T4 *__restrict__ src,
+                                             const int32_t *__restrict__ ids, int64_t p0,
+                                             const double *__restrict__ centres,
+                                             const double *__restrict__ x,
+                                             const double *__restrict__ y,
+                                             const double *__restrict__ z, double *out)
+{
+    using T4 = typename Rec<Real>::T4;
+    __shared__ T4 lds[NCH];
+    __shared__ double red[64][3 * R];
+    const NearItem it = items[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int cnt = it.tgt_cnt;
+    const int T = (cnt + R - 1) / R;
+    const int P = 64 / T;
+    const int ph = lane / T, tl = lane - ph * T;
+    const bool act = ph < P;
+    T4 xt[R];
+    int32_t tid[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        int t = tl + q * T;
+        int tt = (act && t < cnt) ? t : 0;
+        xt[q] = src[it.tgt_start + tt];
+        tid[q] = ids[it.tgt_start + tt];
+    }
+    double S[R][3];
+    Real Pp[R][3];
+#pragma unroll
+    for (int q = 0; q < R; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            S[q][c] = 0.0;
+            Pp[q][c] = (Real)0;
         }
-    }
-    for (int o = T; o < 64; o <<= 1) {
-        S0 += shfl_xor_d(S0, o);
-        S1 += shfl_xor_d(S1, o);
-        S2 += shfl_xor_d(S2, o);
-    }
-    if (ph == 0 && k < p) {
-        phi[0][k] = yk.y * S2 - yk.z * S1;
-        phi[1][k] = yk.z * S0 - yk.x * S2;
-        phi[2][k] = yk.x * S1 - yk.y * S0;
-    }
+    auto body = [&](int k) {
+        if (k < 0) { // chunk boundary: flush the Real partials into FP64
+#pragma unroll
+            for (int q = 0; q < R; q++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    S[q][c] += (double)Pp[q][c];
+                    Pp[q][c] = (Real)0;
+                }
+            return;
+        }
+        T4 s = lds[k];
+#pragma unroll
+        for (int q = 0; q < R; q++) pair_eval(xt[q], s, (Real)0, Pp[q][0], Pp[q][1], Pp[q][2]);
+    };
+    int fill = 0;
+    staged_ranges(ranges, it.a_beg, it.a_end, src, centres, it.node, lds, ph, P, act, body, fill);
     __syncthreads();
-    if (lane < 3) {
-        double b[FSS_MAX_P];
-        for (int q = 0; q < p; q++) b[q] = phi[lane][q];
-        lu_solve_small(p, lu, ipiv, b);
-        double *o = alpha + (int64_t)it.slot * 3 * p + lane * p;
-        for (int q = 0; q < p; q++) o[q] = b[q];
+    if (act)
+        for (int k = ph; k < fill; k += P) body(k);
+    body(-1);
+    // ranges that hold the target itself: skip the self pair by particle id
+    for (int r = it.a_beg; r < it.a_end; r++) {
+        Range rg = ranges[r];
+        if (!(rg.count & FSS_SELF_BIT)) continue;
+        int c0 = rg.count & ~FSS_SELF_BIT;
+        T4 sh = range_shift(src, centres, rg.node, it.node);
+        if (act)
+            for (int j = rg.start + ph; j < rg.start + c0; j += P) {
+                T4 s = shifted(src[j], sh);
+                int32_t sid = ids[j];
+#pragma unroll
+                for (int q = 0; q < R; q++)
+                    if (sid != tid[q]) pair_eval(xt[q], s, (Real)0, Pp[q][0], Pp[q][1], Pp[q][2]);
+            }
     }
-}
-
-// FP32 kernel evaluations, |x - y|^2 / 2 (+ proxy offset) for 1 - x.y,
-// per-range partial sums flushed into FP64.
-__global__ void __launch_bounds__(64) k_proxy32(const ProxyItem *__restrict__ items,
-                                                const int2 *__restrict__ ranges,
-                                                const float4 *__restrict__ src,
-                                                const float *__restrict__ aoff, int64_t p0,
-                                                int p, const double *__restrict__ lu,
-                                                const int32_t *__restrict__ ipiv, double *alpha)
-{
-    __shared__ double phi[3][64];
-    ProxyItem it = items[blockIdx.x];
-    int lane = threadIdx.x;
-    int T = pow2_at_least(p), P = 64 / T;
-    int k = lane & (T - 1), ph = lane / T;
-    int kk = k < p ? k : 0;
-    int64_t pk = p0 + (int64_t)it.node * p + kk;
-    float4 yk = src[pk];
-    float at = aoff[pk - p0];
-    double S0 = 0, S1 = 0, S2 = 0;
-    for (int r = it.b_beg; r < it.b_end; r++) {
-        int2 rg = ranges[r];
-        float P0 = 0.f, P1 = 0.f, P2 = 0.f;
-        for (int j = rg.x + ph; j < rg.x + rg.y; j += P) {
-            float4 s = src[j];
-            float dx = s.x - yk.x, dy = s.y - yk.y, dz = s.z - yk.z;
-            float den = fmaf(0.5f, dx * dx + dy * dy + dz * dz, at);
-            float c = s.w * rcp32(den);
-            P0 = fmaf(c, dx, P0);
-            P1 = fmaf(c, dy, P1);
-            P2 = fmaf(c, dz, P2);
-        }
-        S0 += (double)P0;
-        S1 += (double)P1;
-        S2 += (double)P2;
-    }
-    for (int o = T; o < 64; o <<= 1) {
-        S0 += shfl_xor_d(S0, o);
-        S1 += shfl_xor_d(S1, o);
-        S2 += shfl_xor_d(S2, o);
-    }
-    if (ph == 0 && k < p) {
-        // y x x_j = y x (x_j - y): the cross product uses the same rounded y
-        double y0 = yk.x, y1 = yk.y, y2 = yk.z;
-        phi[0][k] = y1 * S2 - y2 * S1;
-        phi[1][k] = y2 * S0 - y0 * S2;
-        phi[2][k] = y0 * S1 - y1 * S0;
-    }
+    body(-1);
+#pragma unroll
+    for (int q = 0; q < R; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) red[lane][3 * q + c] = S[q][c];
     __syncthreads();
-    if (lane < 3) {
-        double b[FSS_MAX_P];
-        for (int q = 0; q < p; q++) b[q] = phi[lane][q];
-        lu_solve_small(p, lu, ipiv, b);
-        double *o = alpha + (int64_t)it.slot * 3 * p + lane * p;
-        for (int q = 0; q < p; q++) o[q] = b[q];
-    }
-}
-
-// ------------------------------------------------------------------
-// near field (class A) + interpolation of the node's proxy values
-// ------------------------------------------------------------------
-__device__ inline void add_interp(const double *__restrict__ tri_xyz, int node, const double *pt,
-                                  const double *__restrict__ al, int degree, int p, int normalise,
-                                  double &u0, double &u1, double &u2)
-{
-    double c[3], m[FSS_MAX_P];
-    bary_solve(tri_xyz + 9 * (int64_t)node, pt, c);
-    double s = c[0], t = c[1];
-    if (normalise) {
-        double sg = c[0] + c[1] + c[2];
-        s = c[0] / sg;
-        t = c[1] / sg;
-    }
-    monomials(s, t, degree, m);
-    double a0 = 0, a1 = 0, a2 = 0;
-    for (int q = 0; q < p; q++) {
-        a0 = fma(m[q], al[q], a0);
-        a1 = fma(m[q], al[p + q], a1);
-        a2 = fma(m[q], al[2 * p + q], a2);
-    }
-    u0 += a0;
-    u1 += a1;
-    u2 += a2;
-}
-
-__global__ void __launch_bounds__(64) k_near64(const NearItem *__restrict__ items,
-                                               const int2 *__restrict__ ranges,
-                                               const double4 *__restrict__ src,
-                                               const int32_t *__restrict__ ids, int64_t p0,
-                                               const double *__restrict__ alpha,
-                                               const double *__restrict__ tri_xyz, int degree, int p,
-                                               int normalise, double *out)
-{
-    NearItem it = items[blockIdx.x];
-    int lane = threadIdx.x;
-    int T = pow2_at_least(it.tgt_cnt), P = 64 / T;
-    int t = lane & (T - 1), ph = lane / T;
-    int tt = t < it.tgt_cnt ? t : 0;
-    int64_t tpos = it.tgt_start + tt;
-    double4 xt = src[tpos];
-    int32_t tid = ids[tpos];
-    double S0 = 0, S1 = 0, S2 = 0;
-    for (int r = it.a_beg; r < it.a_end; r++) {
-        int2 rg = ranges[r];
-        for (int j = rg.x + ph; j < rg.x + rg.y; j += P) {
-            double4 s = src[j];
-            int32_t sid = j < p0 ? ids[j] : -1;
-            double den = 1.0 - (xt.x * s.x + xt.y * s.y + xt.z * s.z);
-            double c = sid == tid ? 0.0 : s.w * rcp64(den);
-            S0 = fma(c, s.x, S0);
-            S1 = fma(c, s.y, S1);
-            S2 = fma(c, s.z, S2);
+    if (ph == 0) {
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            int t = tl + q * T;
+            if (t >= cnt) continue;
+            double a0 = 0, a1 = 0, a2 = 0;
+            for (int h = 0; h < P; h++) {
+                a0 += red[h * T + tl][3 * q];
+                a1 += red[h * T + tl][3 * q + 1];
+                a2 += red[h * T + tl][3 * q + 2];
+            }
+            double xa[3];
+            abs_pos(xt[q], tid[q], x, y, z, xa);
+            double x0 = xa[0], x1 = xa[1], x2 = xa[2];
+            int64_t pos = it.tgt_start + t;
+            out[pos] = x1 * a2 - x2 * a1;
+            out[p0 + pos] = x2 * a0 - x0 * a2;
+            out[2 * p0 + pos] = x0 * a1 - x1 * a0;
         }
     }
-    for (int o = T; o < 64; o <<= 1) {
-        S0 += shfl_xor_d(S0, o);
-        S1 += shfl_xor_d(S1, o);
-        S2 += shfl_xor_d(S2, o);
-    }
-    if (ph == 0 && t < it.tgt_cnt) {
-        double u0 = xt.y * S2 - xt.z * S1;
-        double u1 = xt.z * S0 - xt.x * S2;
-        double u2 = xt.x * S1 - xt.y * S0;
-        if (it.alpha_slot >= 0) {
-            double pt[3] = {xt.x, xt.y, xt.z};
-            add_interp(tri_xyz, it.node, pt, alpha + (int64_t)it.alpha_slot * 3 * p, degree, p,
-                       normalise, u0, u1, u2);
-        }
-        out[tpos] = u0;
-        out[p0 + tpos] = u1;
-        out[2 * p0 + tpos] = u2;
-    }
-}
-
-__global__ void __launch_bounds__(64) k_near32(const NearItem *__restrict__ items,
-                                               const int2 *__restrict__ ranges,
-                                               const float4 *__restrict__ src,
-                                               const int32_t *__restrict__ ids, int64_t p0,
-                                               const double *__restrict__ x,
-                                               const double *__restrict__ y,
-                                               const double *__restrict__ z,
-                                               const double *__restrict__ alpha,
-                                               const double *__restrict__ tri_xyz, int degree, int p,
-                                               int normalise, double *out)
-{
-    NearItem it = items[blockIdx.x];
-    int lane = threadIdx.x;
-    int T = pow2_at_least(it.tgt_cnt), P = 64 / T;
-    int t = lane & (T - 1), ph = lane / T;
-    int tt = t < it.tgt_cnt ? t : 0;
-    int64_t tpos = it.tgt_start + tt;
-    float4 xt = src[tpos];
-    int32_t tid = ids[tpos];
-    double S0 = 0, S1 = 0, S2 = 0;
-    for (int r = it.a_beg; r < it.a_end; r++) {
-        int2 rg = ranges[r];
-        float P0 = 0.f, P1 = 0.f, P2 = 0.f;
-        for (int j = rg.x + ph; j < rg.x + rg.y; j += P) {
-            float4 s = src[j];
-            int32_t sid = j < p0 ? ids[j] : -1;
-            float dx = s.x - xt.x, dy = s.y - xt.y, dz = s.z - xt.z;
-            float den = 0.5f * (dx * dx + dy * dy + dz * dz);
-            float c = sid == tid ? 0.0f : s.w * rcp32(den);
-            P0 = fmaf(c, dx, P0);
-            P1 = fmaf(c, dy, P1);
-            P2 = fmaf(c, dz, P2);
-        }
-        S0 += (double)P0;
-        S1 += (double)P1;
-        S2 += (double)P2;
-    }
-    for (int o = T; o < 64; o <<= 1) {
-        S0 += shfl_xor_d(S0, o);
-        S1 += shfl_xor_d(S1, o);
-        S2 += shfl_xor_d(S2, o);
-    }
-    if (ph == 0 && t < it.tgt_cnt) {
-        // x x x_j = x x (x_j - x) with the same rounded x used for the differences
-        double x0 = xt.x, x1 = xt.y, x2 = xt.z;
-        double u0 = x1 * S2 - x2 * S1;
-        double u1 = x2 * S0 - x0 * S2;
-        double u2 = x0 * S1 - x1 * S0;
-        double pt[3] = {x[tid], y[tid], z[tid]};
-        if (it.alpha_slot >= 0)
-            add_interp(tri_xyz, it.node, pt, alpha + (int64_t)it.alpha_slot * 3 * p, degree, p,
-                       normalise, u0, u1, u2);
-        out[tpos] = u0;
-        out[p0 + tpos] = u1;
-        out[2 * p0 + tpos] = u2;
-    }
 }
 
 // ------------------------------------------------------------------
-// per-particle reduction over levels + rhs_func scaling (rhs_eval.h:54-55)
+// proxy values (class B): p proxies of a node against one chunk of sources
 // ------------------------------------------------------------------
-__global__ void k_combine(const int32_t *__restrict__ inv_active, int L, int64_t n, int64_t p0,
-                          const double *__restrict__ out, double scale, double omega, double *u,
-                          double *v, double *w, double *dvort)
+template <typename Real, int D>
+__global__ void __launch_bounds__(64) k_proxy(const ProxyItem *__restrict__ items,
+                                              const Range *__restrict__ ranges,
+                                              const typename Rec<Real>::T4 *__restrict__ src,
+                                              const float *__restrict__ aoff, int64_t p0,
+                                              const double *__restrict__ centres,
+                                              const double4 *__restrict__ proxy_xyz, double *phi)
 {
+    using T4 = typename Rec<Real>::T4;
+    constexpr int PP = npts<D>();
+    constexpr int R = PP > 6 ? 2 : 1;
+    __shared__ T4 lds[NCH];
+    __shared__ double red[64][3 * R];
+    const ProxyItem it = items[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int T = (PP + R - 1) / R;
+    const int P = 64 / T;
+    const int ph = lane / T, tl = lane - ph * T;
+    const bool act = ph < P;
+    T4 yk[R];
+    Real at[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        int k = tl + q * T;
+        int kk = (act && k < PP) ? k : 0;
+        int64_t pk = p0 + (int64_t)it.node * PP + kk;
+        yk[q] = src[pk];
+        at[q] = aoff ? (Real)aoff[pk - p0] : (Real)0;
+    }
+    double S[R][3];
+    Real Pp[R][3];
+#pragma unroll
+    for (int q = 0; q < R; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            S[q][c] = 0.0;
+            Pp[q][c] = (Real)0;
+        }
+    auto body = [&](int k) {
+        if (k < 0) {
+#pragma unroll
+            for (int q = 0; q < R; q++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    S[q][c] += (double)Pp[q][c];
+                    Pp[q][c] = (Real)0;
+                }
+            return;
+        }
+        T4 s = lds[k];
+#pragma unroll
+        for (int q = 0; q < R; q++) pair_far(yk[q], s, at[q], Pp[q][0], Pp[q][1], Pp[q][2]);
+    };
+    int fill = 0;
+    staged_ranges(ranges, it.b_beg, it.b_end, src, centres, it.node, lds, ph, P, act, body, fill);
+    __syncthreads();
+    if (act)
+        for (int k = ph; k < fill; k += P) body(k);
+    body(-1);
+#pragma unroll
+    for (int q = 0; q < R; q++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) red[lane][3 * q + c] = S[q][c];
+    __syncthreads();
+    if (ph == 0) {
+        double *o = phi + (int64_t)blockIdx.x * 3 * PP;
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            int k = tl + q * T;
+            if (k >= PP) continue;
+            double a0 = 0, a1 = 0, a2 = 0;
+            for (int h = 0; h < P; h++) {
+                a0 += red[h * T + tl][3 * q];
+                a1 += red[h * T + tl][3 * q + 1];
+                a2 += red[h * T + tl][3 * q + 2];
+            }
+            // y x x_j = y x (x_j - y): absolute FP64 proxy position
+            double4 ya = proxy_xyz[(int64_t)it.node * PP + k];
+            double y0 = ya.x, y1 = ya.y, y2 = ya.z;
+            o[k] = y1 * a2 - y2 * a1;
+            o[PP + k] = y2 * a0 - y0 * a2;
+            o[2 * PP + k] = y0 * a1 - y1 * a0;
+        }
+    }
+}
+
+// sum the chunks of each node in order and solve alpha = V^{-1} phi
+// (dgetrs, fast_rhs.h:217): one thread per (slot, component)
+template <int D>
+__global__ void k_solve(const ProxySlot *__restrict__ slots, int64_t nslot,
+                        const double *__restrict__ phi, const double *__restrict__ lu,
+                        const int32_t *__restrict__ ipiv, double *alpha)
+{
+    constexpr int P = npts<D>();
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * nslot) return;
+    int64_t q = i / 3;
+    int c = (int)(i % 3);
+    ProxySlot sl = slots[q];
+    double b[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) b[k] = 0.0;
+    for (int part = sl.part_beg; part < sl.part_end; part++) {
+        const double *o = phi + (int64_t)part * 3 * P + c * P;
+#pragma unroll
+        for (int k = 0; k < P; k++) b[k] += o[k];
+    }
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+        int r = ipiv[k];
+        if (r != k) {
+            // runtime pivot index: swap through a short select loop
+            double bk = b[k], br = 0.0;
+#pragma unroll
+            for (int t = 0; t < P; t++)
+                if (t == r) br = b[t];
+#pragma unroll
+            for (int t = 0; t < P; t++)
+                if (t == r) b[t] = bk;
+            b[k] = br;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < P; k++)
+#pragma unroll
+        for (int r = k + 1; r < P; r++) b[r] = b[r] - b[k] * lu[r + k * P];
+#pragma unroll
+    for (int k = P - 1; k >= 0; k--) {
+        b[k] = b[k] / lu[k + k * P];
+#pragma unroll
+        for (int r = 0; r < k; r++) b[r] = b[r] - b[k] * lu[r + k * P];
+    }
+    double *o = alpha + q * 3 * P + c * P;
+#pragma unroll
+    for (int k = 0; k < P; k++) o[k] = b[k];
+}
+
+// ------------------------------------------------------------------
+// per-particle reduction over levels, interpolation of the proxy values of
+// every node on the particle's path, rhs_func scaling (rhs_eval.h:54-55)
+// ------------------------------------------------------------------
+template <int D>
+__global__ void k_combine(const int32_t *__restrict__ inv_active, const int32_t *__restrict__ nodeof,
+                          const int32_t *__restrict__ node_slot, const double *__restrict__ alpha,
+                          const double *__restrict__ tri_xyz, const double *__restrict__ x,
+                          const double *__restrict__ y, const double *__restrict__ z, int L,
+                          int64_t n, int64_t p0, const double *__restrict__ out, int normalise,
+                          double scale, double omega, double *u, double *v, double *w,
+                          double *dvort)
+{
+    constexpr int P = npts<D>();
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    double pt[3] = {x[i], y[i], z[i]};
     double a0 = 0, a1 = 0, a2 = 0;
     for (int l = 0; l < L; l++) {
         int32_t pos = inv_active[(int64_t)l * n + i];
@@ -483,6 +620,30 @@ __global__ void k_combine(const int32_t *__restrict__ inv_active, int L, int64_t
             a0 += out[pos];
             a1 += out[p0 + pos];
             a2 += out[2 * p0 + pos];
+        }
+        int32_t g = nodeof[(int64_t)l * n + i];
+        int32_t sl = g >= 0 ? node_slot[g] : -1;
+        if (sl >= 0) {
+            double c[3], m[P];
+            bary_solve(tri_xyz + 9 * (int64_t)g, pt, c);
+            double s = c[0], t = c[1];
+            if (normalise) {
+                double sg = c[0] + c[1] + c[2];
+                s = c[0] / sg;
+                t = c[1] / sg;
+            }
+            monomials<D>(s, t, m);
+            const double *al = alpha + (int64_t)sl * 3 * P;
+            double b0 = 0, b1 = 0, b2 = 0;
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                b0 = fma(m[q], al[q], b0);
+                b1 = fma(m[q], al[P + q], b1);
+                b2 = fma(m[q], al[2 * P + q], b2);
+            }
+            a0 += b0;
+            a1 += b1;
+            a2 += b2;
         }
     }
     a0 *= scale;
@@ -497,6 +658,85 @@ __global__ void k_combine(const int32_t *__restrict__ inv_active, int L, int64_t
 // ------------------------------------------------------------------
 // host launcher
 // ------------------------------------------------------------------
+template <int D>
+static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity *out, hipEvent_t *ev)
+{
+    Tree &T = ctx->tree;
+    hipStream_t s = ctx->stream;
+    constexpr int p = npts<D>();
+    const bool mixed = ctx->cfg.precision == FSS_MIXED;
+    const bool bltc = ctx->cfg.eval_mode == FSS_EVAL_BLTC;
+    const int64_t p0 = T.p0, nprox = T.nn * p, nsrc = p0 + nprox;
+    const double scale = -1.0 / (4.0 * M_PI);
+    FSS_TRY(ctx->out3.ensure(sizeof(double) * 3 * std::max<int64_t>(1, p0)));
+    FSS_TRY(ctx->phi.ensure(sizeof(double) * 3 * p * std::max<int64_t>(1, T.n_proxy)));
+    FSS_TRY(ctx->alpha.ensure(sizeof(double) * 3 * p * std::max<int64_t>(1, T.n_slot)));
+    if (ev) FSS_HIP(hipEventRecord(ev[0], s));
+    unsigned gs = (unsigned)div_up(nsrc, 256);
+    if (!mixed) {
+        FSS_TRY(ctx->src64.ensure(sizeof(double4) * nsrc));
+        double4 *src = ctx->src64.as<double4>();
+        hipLaunchKernelGGL(k_gather64, dim3(gs), dim3(256), 0, s, pp->x, pp->y, pp->z, pp->vort,
+                           pp->area, T.ids.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, src);
+        if (bltc && T.n_charge > 0)
+            hipLaunchKernelGGL((k_upward<double, D>), dim3((unsigned)T.n_charge), dim3(256), 0, s,
+                               T.charge_nodes.as<int32_t>(), T.offsets.as<int32_t>(),
+                               T.counts.as<int32_t>(), T.tri_xyz.as<double>(), pp->x, pp->y, pp->z,
+                               pp->vort, pp->area, T.ids.as<int32_t>(), ctx->d_vinv.as<double>(), p0,
+                               reinterpret_cast<double *>(src));
+        if (T.n_proxy > 0)
+            hipLaunchKernelGGL((k_proxy<double, D>), dim3((unsigned)T.n_proxy), dim3(64), 0, s,
+                               T.proxy_items.as<ProxyItem>(), T.ranges_b.as<Range>(), src,
+                               (const float *)nullptr, p0, T.tri_info.as<double>(),
+                               T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
+        if (ev) FSS_HIP(hipEventRecord(ev[1], s));
+        if (T.n_near > 0)
+            hipLaunchKernelGGL((k_near<double, 2>), dim3((unsigned)T.n_near), dim3(64), 0, s,
+                               T.near_items.as<NearItem>(), T.ranges_a.as<Range>(), src,
+                               T.ids.as<int32_t>(), p0, T.tri_info.as<double>(), pp->x, pp->y,
+                               pp->z, ctx->out3.as<double>());
+        if (ev) FSS_HIP(hipEventRecord(ev[2], s));
+    } else {
+        FSS_TRY(ctx->src32.ensure(sizeof(float4) * nsrc + sizeof(float) * nprox));
+        float4 *src = ctx->src32.as<float4>();
+        float *aoff = reinterpret_cast<float *>(src + nsrc);
+        hipLaunchKernelGGL(k_gather32, dim3(gs), dim3(256), 0, s, pp->x, pp->y, pp->z, pp->vort,
+                           pp->area, T.ids.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, p,
+                           T.posnode.as<int32_t>(), T.tri_info.as<double>(), bltc ? 0 : 1, src,
+                           aoff);
+        if (bltc && T.n_charge > 0)
+            hipLaunchKernelGGL((k_upward<float, D>), dim3((unsigned)T.n_charge), dim3(256), 0, s,
+                               T.charge_nodes.as<int32_t>(), T.offsets.as<int32_t>(),
+                               T.counts.as<int32_t>(), T.tri_xyz.as<double>(), pp->x, pp->y, pp->z,
+                               pp->vort, pp->area, T.ids.as<int32_t>(), ctx->d_vinv.as<double>(), p0,
+                               reinterpret_cast<float *>(src));
+        if (T.n_proxy > 0)
+            hipLaunchKernelGGL((k_proxy<float, D>), dim3((unsigned)T.n_proxy), dim3(64), 0, s,
+                               T.proxy_items.as<ProxyItem>(), T.ranges_b.as<Range>(), src,
+                               (const float *)aoff, p0, T.tri_info.as<double>(),
+                               T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
+        if (ev) FSS_HIP(hipEventRecord(ev[1], s));
+        if (T.n_near > 0)
+            hipLaunchKernelGGL((k_near<float, 2>), dim3((unsigned)T.n_near), dim3(64), 0, s,
+                               T.near_items.as<NearItem>(), T.ranges_a.as<Range>(), src,
+                               T.ids.as<int32_t>(), p0, T.tri_info.as<double>(), pp->x, pp->y,
+                               pp->z, ctx->out3.as<double>());
+        if (ev) FSS_HIP(hipEventRecord(ev[2], s));
+    }
+    if (T.n_slot > 0)
+        hipLaunchKernelGGL((k_solve<D>), dim3((unsigned)div_up(3 * T.n_slot, 128)), dim3(128), 0, s,
+                           T.proxy_slots.as<ProxySlot>(), T.n_slot, ctx->phi.as<double>(),
+                           ctx->d_lu.as<double>(), ctx->d_ipiv.as<int32_t>(), ctx->alpha.as<double>());
+    hipLaunchKernelGGL((k_combine<D>), dim3((unsigned)div_up(T.n, 256)), dim3(256), 0, s,
+                       T.inv_active.as<int32_t>(), T.nodeof.as<int32_t>(), T.node_slot.as<int32_t>(),
+                       ctx->alpha.as<double>(), T.tri_xyz.as<double>(), pp->x, pp->y, pp->z, T.L, T.n,
+                       p0, ctx->out3.as<double>(), bltc ? 1 : 0, scale, ctx->cfg.omega, out->u,
+                       out->v, out->w, out->dvort);
+    if (ev) FSS_HIP(hipEventRecord(ev[3], s));
+    FSS_HIP(hipGetLastError());
+    return FSS_OK;
+}
+
 fss_status velocity_tree(fss_ctx *ctx, const fss_particles *pp, fss_velocity *out)
 {
     Tree &T = ctx->tree;
@@ -508,78 +748,25 @@ fss_status velocity_tree(fss_ctx *ctx, const fss_particles *pp, fss_velocity *ou
         set_error("particle count differs from the tree's");
         return FSS_ERR_INVALID;
     }
-    hipStream_t s = ctx->stream;
-    const int p = ctx->interp.p, degree = ctx->interp.degree;
-    const bool mixed = ctx->cfg.precision == FSS_MIXED;
-    const bool bltc = ctx->cfg.eval_mode == FSS_EVAL_BLTC;
-    const int64_t p0 = T.p0, nprox = T.nn * p, nsrc = p0 + nprox;
-    const double scale = -1.0 / (4.0 * M_PI);
-    FSS_TRY(ctx->out3.ensure(sizeof(double) * 3 * std::max<int64_t>(1, p0)));
-    FSS_TRY(ctx->alpha.ensure(sizeof(double) * 3 * p * std::max<int64_t>(1, T.n_proxy)));
-    hipEvent_t *ev = ctx->timing.ev;
-    if (!ctx->capturing) FSS_HIP(hipEventRecord(ev[0], s));
-    unsigned gs = (unsigned)div_up(nsrc, 256);
-    if (!mixed) {
-        FSS_TRY(ctx->src64.ensure(sizeof(double4) * nsrc));
-        hipLaunchKernelGGL(k_gather64, dim3(gs), dim3(256), 0, s, pp->x, pp->y, pp->z, pp->vort,
-                           pp->area, T.ids.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox,
-                           ctx->src64.as<double4>());
-        if (bltc && T.n_charge > 0)
-            hipLaunchKernelGGL(k_upward<double>, dim3((unsigned)T.n_charge), dim3(64), 0, s,
-                               T.charge_nodes.as<int32_t>(), T.n_charge, T.offsets.as<int32_t>(),
-                               T.counts.as<int32_t>(), T.tri_xyz.as<double>(), pp->x, pp->y, pp->z,
-                               pp->vort, pp->area, T.ids.as<int32_t>(), ctx->d_vinv.as<double>(),
-                               degree, p, p0, ctx->src64.as<double>());
-        if (T.n_proxy > 0)
-            hipLaunchKernelGGL(k_proxy64, dim3((unsigned)T.n_proxy), dim3(64), 0, s,
-                               T.proxy_items.as<ProxyItem>(), T.ranges_b.as<int2>(),
-                               ctx->src64.as<double4>(), p0, p, ctx->d_lu.as<double>(),
-                               ctx->d_ipiv.as<int32_t>(), ctx->alpha.as<double>());
-        if (!ctx->capturing) FSS_HIP(hipEventRecord(ev[1], s));
-        if (T.n_near > 0)
-            hipLaunchKernelGGL(k_near64, dim3((unsigned)T.n_near), dim3(64), 0, s,
-                               T.near_items.as<NearItem>(), T.ranges_a.as<int2>(),
-                               ctx->src64.as<double4>(), T.ids.as<int32_t>(), p0,
-                               ctx->alpha.as<double>(), T.tri_xyz.as<double>(), degree, p,
-                               bltc ? 1 : 0, ctx->out3.as<double>());
-        if (!ctx->capturing) FSS_HIP(hipEventRecord(ev[2], s));
-        hipLaunchKernelGGL(k_combine, dim3((unsigned)div_up(T.n, 256)), dim3(256), 0, s,
-                           T.inv_active.as<int32_t>(), T.L, T.n, p0, ctx->out3.as<double>(), scale,
-                           ctx->cfg.omega, out->u, out->v, out->w, out->dvort);
-    } else {
-        FSS_TRY(ctx->src32.ensure(sizeof(float4) * nsrc + sizeof(float) * nprox));
-        float4 *src = ctx->src32.as<float4>();
-        float *aoff = reinterpret_cast<float *>(src + nsrc);
-        hipLaunchKernelGGL(k_gather32, dim3(gs), dim3(256), 0, s, pp->x, pp->y, pp->z, pp->vort,
-                           pp->area, T.ids.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox,
-                           bltc ? 0 : 1, src, aoff);
-        if (bltc && T.n_charge > 0)
-            hipLaunchKernelGGL(k_upward<float>, dim3((unsigned)T.n_charge), dim3(64), 0, s,
-                               T.charge_nodes.as<int32_t>(), T.n_charge, T.offsets.as<int32_t>(),
-                               T.counts.as<int32_t>(), T.tri_xyz.as<double>(), pp->x, pp->y, pp->z,
-                               pp->vort, pp->area, T.ids.as<int32_t>(), ctx->d_vinv.as<double>(),
-                               degree, p, p0, reinterpret_cast<float *>(src));
-        if (T.n_proxy > 0)
-            hipLaunchKernelGGL(k_proxy32, dim3((unsigned)T.n_proxy), dim3(64), 0, s,
-                               T.proxy_items.as<ProxyItem>(), T.ranges_b.as<int2>(), src, aoff,
-                               p0, p, ctx->d_lu.as<double>(),
-                               ctx->d_ipiv.as<int32_t>(), ctx->alpha.as<double>());
-        if (!ctx->capturing) FSS_HIP(hipEventRecord(ev[1], s));
-        if (T.n_near > 0)
-            hipLaunchKernelGGL(k_near32, dim3((unsigned)T.n_near), dim3(64), 0, s,
-                               T.near_items.as<NearItem>(), T.ranges_a.as<int2>(), src,
-                               T.ids.as<int32_t>(), p0, pp->x, pp->y, pp->z, ctx->alpha.as<double>(),
-                               T.tri_xyz.as<double>(), degree, p, bltc ? 1 : 0,
-                               ctx->out3.as<double>());
-        if (!ctx->capturing) FSS_HIP(hipEventRecord(ev[2], s));
-        hipLaunchKernelGGL(k_combine, dim3((unsigned)div_up(T.n, 256)), dim3(256), 0, s,
-                           T.inv_active.as<int32_t>(), T.L, T.n, p0, ctx->out3.as<double>(), scale,
-                           ctx->cfg.omega, out->u, out->v, out->w, out->dvort);
+    hipEvent_t *ev = nullptr;
+    if (!ctx->capturing) {
+        timing_begin(ctx->timing);
+        ev = ctx->timing.slot();
     }
-    if (!ctx->capturing) FSS_HIP(hipEventRecord(ev[3], s));
-    if (!ctx->capturing) ctx->timing.valid = true;
-    FSS_HIP(hipGetLastError());
-    return FSS_OK;
+    fss_status st;
+    switch (ctx->interp.degree) {
+    case 1: st = run_degree<1>(ctx, pp, out, ev); break;
+    case 2: st = run_degree<2>(ctx, pp, out, ev); break;
+    case 3: st = run_degree<3>(ctx, pp, out, ev); break;
+    case 4: st = run_degree<4>(ctx, pp, out, ev); break;
+    case 5: st = run_degree<5>(ctx, pp, out, ev); break;
+    case 6: st = run_degree<6>(ctx, pp, out, ev); break;
+    case 7: st = run_degree<7>(ctx, pp, out, ev); break;
+    case 8: st = run_degree<8>(ctx, pp, out, ev); break;
+    default: set_error("unsupported degree"); return FSS_ERR_INVALID;
+    }
+    if (st == FSS_OK && !ctx->capturing) ctx->timing.valid = true;
+    return st;
 }
 
 } // namespace fss

@@ -109,30 +109,46 @@ fss_status fss_grid_icosahedral(int level, double *x, double *y, double *z, int3
 {
     if (level < 0 || level > 12 || !x || !y || !z) return set_error("bad grid arguments"), FSS_ERR_INVALID;
     try {
+        // icos_init dedupes new vertices by exact coordinate equality
+        // (check_in_vec, helpers.h:224-229).  A midpoint's bits depend only on
+        // its edge ((a + b) * 0.5 is symmetric), and distinct edges give
+        // distinct points, so an edge-keyed table yields the same vertex list
+        // in the same order.
         const double radius = 1.0;
+        const size_t nv_final = 10 * ((size_t)1 << (2 * level)) + 2;
         std::vector<double> verts;
-        std::unordered_map<Key, int32_t, KeyHash> where;
-        auto add = [&](const double *v) -> int32_t {
-            Key k = key_of(v);
-            auto it = where.find(k);
-            if (it != where.end()) return it->second;
-            int32_t id = (int32_t)(verts.size() / 3);
-            verts.insert(verts.end(), v, v + 3);
-            where.emplace(k, id);
-            return id;
+        verts.reserve(3 * nv_final);
+        size_t cap = 16;
+        while (cap < 2 * 3 * nv_final) cap <<= 1;
+        std::vector<uint64_t> ekey(cap, ~0ull);
+        std::vector<int32_t> eval(cap);
+        auto add_edge = [&](int32_t a, int32_t b, const double *v) -> int32_t {
+            uint64_t k = a < b ? ((uint64_t)a << 32 | (uint32_t)b) : ((uint64_t)b << 32 | (uint32_t)a);
+            uint64_t h = (k * 0x9e3779b97f4a7c15ull) >> 20;
+            for (;; h++) {
+                size_t s = h & (cap - 1);
+                if (ekey[s] == ~0ull) {
+                    ekey[s] = k;
+                    eval[s] = (int32_t)(verts.size() / 3);
+                    verts.insert(verts.end(), v, v + 3);
+                    return eval[s];
+                }
+                if (ekey[s] == k) return eval[s];
+            }
         };
         fssg::IcosBase b;
         fssg::icos_base(&b);
         for (int i = 0; i < 12; i++) {
             double v[3];
             fssg::project_to_sphere_2(b.v[i], radius, v);
-            add(v);
+            verts.insert(verts.end(), v, v + 3);
         }
         std::vector<int32_t> tv(60);
         for (int i = 0; i < 20; i++)
             for (int m = 0; m < 3; m++) tv[3 * i + m] = b.f[i][m] - 1;
         for (int l = 0; l < level; l++) {
             std::vector<int32_t> nt(tv.size() * 4);
+            std::fill(ekey.begin(), ekey.end(), ~0ull);
             for (size_t j = 0; j < tv.size() / 3; j++) {
                 int32_t iv1 = tv[3 * j], iv2 = tv[3 * j + 1], iv3 = tv[3 * j + 2];
                 double v1[3], v2[3], v3[3];
@@ -142,7 +158,8 @@ fss_status fss_grid_icosahedral(int level, double *x, double *y, double *z, int3
                 double v12[3], v23[3], v31[3];
                 fssg::midpoints(v1, v2, v3, radius, v12, v23, v31);
                 // insertion order of check_in_vec hits: v12, v31, v23 (icos_utils.h:91-105)
-                int32_t iv12 = add(v12), iv13 = add(v31), iv23 = add(v23);
+                int32_t iv12 = add_edge(iv1, iv2, v12), iv13 = add_edge(iv3, iv1, v31),
+                        iv23 = add_edge(iv2, iv3, v23);
                 int32_t *c = &nt[12 * j];
                 c[0] = iv1; c[1] = iv13; c[2] = iv12;
                 c[3] = iv3; c[4] = iv23; c[5] = iv13;

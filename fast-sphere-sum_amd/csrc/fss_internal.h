@@ -61,23 +61,51 @@ struct DevBuf {
 };
 
 // One work item of the direct-target ("near") kernel: up to 64 targets of one
-// node, a list of source ranges, and an optional interpolant to add.
+// node and the list of its class-A source ranges.
 struct NearItem {
     int32_t tgt_start; // position in the level-ordered source array
     int32_t tgt_cnt;   // 1..64
     int32_t a_beg;     // [a_beg, a_end) into the sorted range list of class A
     int32_t a_end;
-    int32_t alpha_slot; // >= 0: add interpolant alpha[slot] (class-B work of this node)
-    int32_t node;       // global node index (geometry of the target triangle)
+    int32_t node;      // global index of the target node
+    int32_t pad;
 };
 
-// One work item of the proxy kernel: the p proxies of one node.
+// A source range: records [start, start + count) of the level-ordered source
+// array, all belonging to (or being the proxies of) node `node`.  In mixed
+// precision the records hold FP32 offsets from that node's centre.
+struct Range {
+    int32_t start;
+    int32_t count; // may carry FSS_SELF_BIT
+    int32_t node;
+    int32_t pad;
+};
+
+// One work item of the proxy kernel: the p proxies of one node against one
+// chunk of that node's class-B source ranges (long lists are split so that
+// every item carries a bounded amount of work).
 struct ProxyItem {
     int32_t node;
     int32_t b_beg; // [b_beg, b_end) into the sorted range list of class B
     int32_t b_end;
-    int32_t slot; // alpha output slot
+    int32_t slot; // interpolant slot of the node
 };
+
+// The chunks [part_beg, part_end) of one node's proxy work, summed in order
+// before the interpolant is solved.
+struct ProxySlot {
+    int32_t node;
+    int32_t part_beg;
+    int32_t part_end;
+    int32_t pad;
+};
+
+// A source range (start, count) whose count carries this bit when the source
+// node contains the target node (ancestor or self): the self pair must be
+// skipped there (rhs_eval.h:20, fast_rhs.h:107).
+#define FSS_SELF_BIT (1 << 30)
+// Sources per proxy work item before a node's class-B list is split.
+#define FSS_PROXY_CHUNK 4096
 
 // Interpolation constants (host copy + device copy).
 struct Interp {
@@ -108,23 +136,42 @@ struct Tree {
     DevBuf offsets;  // int32 [nn] global start in ids
     DevBuf ids;      // int32 [p0]
     DevBuf inter;    // int32 [7][ninter] AoS
-    DevBuf inv_active; // int32 [L][n]: position of particle in an active node or -1
+    DevBuf inv_active; // int32 [L][n]: position of a particle in a class-A node, or -1
+    DevBuf nodeof;     // int32 [L][n]: global node of a particle per level, or -1 (dropped)
+    DevBuf node_slot;  // int32 [nn]: interpolant slot of a node with class-B work, or -1
 
     // evaluation plan
-    int64_t n_near = 0, n_proxy = 0, n_ra = 0, n_rb = 0, n_charge = 0;
+    int64_t n_near = 0, n_proxy = 0, n_slot = 0, n_ra = 0, n_rb = 0, n_charge = 0;
     DevBuf near_items;  // NearItem[n_near]
     DevBuf proxy_items; // ProxyItem[n_proxy]
-    DevBuf ranges_a;    // int2 (start, count) [n_ra]
-    DevBuf ranges_b;    // int2 [n_rb]
+    DevBuf proxy_slots; // ProxySlot[n_slot]
+    DevBuf ranges_a;    // Range [n_ra]
+    DevBuf ranges_b;    // Range [n_rb]
+    DevBuf posnode;     // int32 [p0]: node of every level-ordered position
     DevBuf charge_nodes; // int32 [n_charge] nodes whose proxies carry charges (BLTC)
     DevBuf proxy_xyz;   // double4 [nn * p] static proxy positions (w unused)
     int64_t pairs_near = 0, pairs_far = 0;
 };
 
+// HIP events around each velocity call: a ring of RING slots, each holding
+// {stage start, main kernel start, main kernel end, stage end}.  Recording
+// never synchronizes; the mean over the slots since the last reset is read
+// with fss_timing_mean().
 struct Timing {
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    static constexpr int RING = 256;
+    hipEvent_t ev[RING][4] = {};
+    int64_t calls = 0;  // calls recorded since the last reset
+    int cur = -1;       // slot of the current call
     bool valid = false;
+    hipEvent_t *slot() { return ev[cur]; }
 };
+
+// begin a timed call: picks the next ring slot
+inline void timing_begin(Timing &t)
+{
+    t.cur = (int)(t.calls % Timing::RING);
+    t.calls++;
+}
 
 } // namespace fss
 
@@ -144,7 +191,8 @@ struct fss_ctx {
     fss::DevBuf src32;   // float4
     fss::DevBuf srcid;   // int32 ids for the direct sum (unused for tree: ids reused)
     fss::DevBuf out3;    // 3 doubles SoA [p0] per-position partial velocities
-    fss::DevBuf alpha;   // 3*p doubles per proxy item
+    fss::DevBuf phi;     // 3*p doubles per proxy work item (partial proxy values)
+    fss::DevBuf alpha;   // 3*p doubles per interpolant slot
     fss::DevBuf rk_k;    // 4 stages x 4 fields x n
     fss::DevBuf rk_y;    // stage state 4 x n
     int64_t rk_n = 0;

@@ -246,7 +246,7 @@ __global__ void k_plan_keys(const int32_t *__restrict__ inter, int64_t m, uint32
 
 __global__ void k_plan_ranges(const int32_t *__restrict__ inter, const int32_t *__restrict__ sorted_idx,
                               int64_t m, const int32_t *__restrict__ offsets, int bltc, int p,
-                              int64_t p0, int2 *ranges, unsigned long long *pair_acc)
+                              int64_t p0, Range *ranges, unsigned long long *pair_acc)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
@@ -254,19 +254,35 @@ __global__ void k_plan_ranges(const int32_t *__restrict__ inter, const int32_t *
     int type = o[6];
     int64_t gs = level_offset(o[1]) + o[3];
     bool proxy_src = bltc && (type == 1 || type == 3);
-    int2 r;
+    Range r;
+    r.node = (int32_t)gs;
+    r.pad = 0;
     if (proxy_src) {
-        r.x = (int32_t)(p0 + gs * p);
-        r.y = p;
+        r.start = (int32_t)(p0 + gs * p);
+        r.count = p;
     } else {
-        r.x = offsets[gs];
-        r.y = o[5];
+        r.start = offsets[gs];
+        r.count = o[5];
     }
-    ranges[k] = r;
     // pair accounting: near = direct-target evaluations, far = proxy-target
     bool clsA = (type == 0 || type == 1);
     unsigned long long tgt = clsA ? (unsigned long long)o[4] : (unsigned long long)p;
-    atomicAdd(&pair_acc[clsA ? 0 : 1], tgt * (unsigned long long)r.y);
+    atomicAdd(&pair_acc[clsA ? 0 : 1], tgt * (unsigned long long)r.count);
+    // particle sources of a node that contains the target node: the self pair
+    // lies in this range
+    int lt = o[0], ls = o[1];
+    if (clsA && !proxy_src && ls <= lt && (o[2] >> (2 * (lt - ls))) == o[3]) r.count |= FSS_SELF_BIT;
+    ranges[k] = r;
+}
+
+// node of every level-ordered position (one block per node)
+__global__ void k_plan_posnode(const int32_t *__restrict__ counts, const int32_t *__restrict__ offsets,
+                               uint32_t nn, int32_t *posnode)
+{
+    int64_t g = blockIdx.x;
+    if (g >= nn) return;
+    int32_t c = counts[g], off = offsets[g];
+    for (int32_t t = threadIdx.x; t < c; t += blockDim.x) posnode[off + t] = (int32_t)g;
 }
 
 __global__ void k_plan_segments(const uint32_t *__restrict__ keys, int64_t m, uint32_t nn,
@@ -289,7 +305,7 @@ __global__ void k_plan_nodes(const int32_t *__restrict__ beg, const int32_t *__r
     bool a = end[g] > beg[g];
     bool b = end[nn + g] > beg[nn + g];
     int32_t c = counts[g];
-    nchunks[g] = (a || b) ? (int32_t)((c + 63) / 64) : 0;
+    nchunks[g] = a ? (int32_t)((c + 63) / 64) : 0;
     hasB[g] = b ? 1 : 0;
 }
 
@@ -297,34 +313,44 @@ __global__ void k_plan_items(const int32_t *__restrict__ beg, const int32_t *__r
                              const int32_t *__restrict__ counts, const int32_t *__restrict__ offsets,
                              const int32_t *__restrict__ chunk_pos, const int32_t *__restrict__ bslot,
                              const int32_t *__restrict__ hasB, uint32_t nn, int64_t nA,
-                             NearItem *near, ProxyItem *prox)
+                             NearItem *near, ProxyItem *prox, int32_t *node_slot)
 {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nn) return;
     bool a = end[g] > beg[g];
     bool b = hasB[g] != 0;
-    if (!(a || b)) return;
-    int32_t c = counts[g], off = offsets[g];
-    int32_t slot = b ? bslot[g] : -1;
+    node_slot[g] = b ? bslot[g] : -1;
     if (b) {
         ProxyItem it;
         it.node = (int32_t)g;
         it.b_beg = (int32_t)(beg[nn + g] - nA);
         it.b_end = (int32_t)(end[nn + g] - nA);
-        it.slot = slot;
-        prox[slot] = it;
+        it.slot = bslot[g];
+        prox[bslot[g]] = it;
     }
+    if (!a) return;
+    int32_t c = counts[g], off = offsets[g];
     int32_t nch = (c + 63) / 64;
     for (int32_t q = 0; q < nch; q++) {
         NearItem it;
         it.tgt_start = off + 64 * q;
         it.tgt_cnt = min(64, c - 64 * q);
-        it.a_beg = a ? beg[g] : 0;
-        it.a_end = a ? end[g] : 0;
-        it.alpha_slot = slot;
+        it.a_beg = beg[g];
+        it.a_end = end[g];
         it.node = (int32_t)g;
+        it.pad = 0;
         near[chunk_pos[g] + q] = it;
     }
+}
+
+// node of every particle per level (-1 where the particle is not on a list)
+__global__ void k_plan_nodeof(const int32_t *__restrict__ loc, const uint8_t *__restrict__ found,
+                              int L, int64_t n, int32_t *nodeof)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)L * n) return;
+    int l = (int)(i / n);
+    nodeof[i] = found[i] ? (int32_t)(level_offset(l) + loc[i]) : -1;
 }
 
 // inverse map for the final per-particle reduction: for every node with any
@@ -429,7 +455,8 @@ static fss_status d2h(void *dst, const void *src, size_t bytes, hipStream_t s)
 void tree_release(Tree &t)
 {
     DevBuf *bufs[] = {&t.tri_xyz, &t.tri_info, &t.loc, &t.found, &t.counts, &t.offsets, &t.ids,
-                      &t.inter, &t.inv_active, &t.near_items, &t.proxy_items, &t.ranges_a,
+                      &t.inter, &t.inv_active, &t.nodeof, &t.node_slot, &t.proxy_slots, &t.posnode,
+                      &t.near_items, &t.proxy_items, &t.ranges_a,
                       &t.ranges_b, &t.charge_nodes, &t.proxy_xyz};
     for (DevBuf *b : bufs) b->release();
     t.built = false;
@@ -623,10 +650,10 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
         unsigned long long hA;
         FSS_TRY(d2h(&hA, acc.as<unsigned long long>() + 2, 8, s));
         nA = (int64_t)hA;
-        FSS_TRY(T.ranges_a.ensure(sizeof(int2) * m));
+        FSS_TRY(T.ranges_a.ensure(sizeof(Range) * m));
         hipLaunchKernelGGL(k_plan_ranges, dim3(gb), dim3(256), 0, s, T.inter.as<int32_t>(),
                            vals2.as<int32_t>(), m, T.offsets.as<int32_t>(), bltc ? 1 : 0, p, T.p0,
-                           T.ranges_a.as<int2>(), acc.as<unsigned long long>());
+                           T.ranges_a.as<Range>(), acc.as<unsigned long long>());
         hipLaunchKernelGGL(k_plan_segments, dim3(gb), dim3(256), 0, s, keys2.as<uint32_t>(), m, nn,
                            beg, end);
     }
@@ -656,19 +683,64 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
         FSS_TRY(d2h(&t4[2], bslot.as<int32_t>() + T.nn - 1, 4, s));
         FSS_TRY(d2h(&t4[3], hasB.as<int32_t>() + T.nn - 1, 4, s));
         T.n_near = (int64_t)t4[0] + t4[1];
-        T.n_proxy = (int64_t)t4[2] + t4[3];
+        T.n_slot = (int64_t)t4[2] + t4[3];
     }
     FSS_TRY(T.near_items.ensure(sizeof(NearItem) * std::max<int64_t>(1, T.n_near)));
-    FSS_TRY(T.proxy_items.ensure(sizeof(ProxyItem) * std::max<int64_t>(1, T.n_proxy)));
+    DevBuf prox_one;
+    FSS_TRY(prox_one.ensure(sizeof(ProxyItem) * std::max<int64_t>(1, T.n_slot)));
+    FSS_TRY(T.node_slot.ensure(sizeof(int32_t) * T.nn));
     // class-B ranges follow class-A ranges in the sorted list
-    FSS_TRY(T.ranges_b.ensure(sizeof(int2) * std::max<int64_t>(1, T.n_rb)));
+    FSS_TRY(T.ranges_b.ensure(sizeof(Range) * std::max<int64_t>(1, T.n_rb)));
     if (T.n_rb > 0)
-        FSS_HIP(hipMemcpyAsync(T.ranges_b.p, T.ranges_a.as<int2>() + nA, sizeof(int2) * T.n_rb,
+        FSS_HIP(hipMemcpyAsync(T.ranges_b.p, T.ranges_a.as<Range>() + nA, sizeof(Range) * T.n_rb,
                                hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_plan_items, dim3(gn), dim3(256), 0, s, beg, end, T.counts.as<int32_t>(),
                        T.offsets.as<int32_t>(), chpos.as<int32_t>(), bslot.as<int32_t>(),
                        hasB.as<int32_t>(), nn, nA, T.near_items.as<NearItem>(),
-                       T.proxy_items.as<ProxyItem>());
+                       prox_one.as<ProxyItem>(), T.node_slot.as<int32_t>());
+    // split long class-B lists into chunks of about FSS_PROXY_CHUNK sources
+    // (host pass over the per-node lists; order inside a node is kept)
+    {
+        std::vector<ProxyItem> one(T.n_slot);
+        std::vector<Range> rb(T.n_rb);
+        if (T.n_slot > 0) FSS_TRY(d2h(one.data(), prox_one.p, sizeof(ProxyItem) * T.n_slot, s));
+        if (T.n_rb > 0) FSS_TRY(d2h(rb.data(), T.ranges_b.p, sizeof(Range) * T.n_rb, s));
+        std::vector<ProxyItem> parts;
+        std::vector<ProxySlot> slots(T.n_slot);
+        for (int64_t q = 0; q < T.n_slot; q++) {
+            const ProxyItem &it = one[q];
+            slots[q].node = it.node;
+            slots[q].part_beg = (int32_t)parts.size();
+            slots[q].pad = 0;
+            int32_t b = it.b_beg;
+            int64_t acc = 0;
+            for (int32_t r = it.b_beg; r < it.b_end; r++) {
+                acc += rb[r].count & ~FSS_SELF_BIT;
+                if (acc >= FSS_PROXY_CHUNK || r + 1 == it.b_end) {
+                    parts.push_back(ProxyItem{it.node, b, r + 1, (int32_t)q});
+                    b = r + 1;
+                    acc = 0;
+                }
+            }
+            slots[q].part_end = (int32_t)parts.size();
+        }
+        T.n_proxy = (int64_t)parts.size();
+        FSS_TRY(T.proxy_items.ensure(sizeof(ProxyItem) * std::max<int64_t>(1, T.n_proxy)));
+        FSS_TRY(T.proxy_slots.ensure(sizeof(ProxySlot) * std::max<int64_t>(1, T.n_slot)));
+        if (T.n_proxy > 0)
+            FSS_HIP(hipMemcpyAsync(T.proxy_items.p, parts.data(), sizeof(ProxyItem) * T.n_proxy,
+                                   hipMemcpyHostToDevice, s));
+        if (T.n_slot > 0)
+            FSS_HIP(hipMemcpyAsync(T.proxy_slots.p, slots.data(), sizeof(ProxySlot) * T.n_slot,
+                                   hipMemcpyHostToDevice, s));
+        FSS_HIP(hipStreamSynchronize(s));
+    }
+    FSS_TRY(T.nodeof.ensure(sizeof(int32_t) * (size_t)L * n));
+    hipLaunchKernelGGL(k_plan_nodeof, dim3((unsigned)div_up((int64_t)L * n, 256)), dim3(256), 0, s,
+                       T.loc.as<int32_t>(), T.found.as<uint8_t>(), L, n, T.nodeof.as<int32_t>());
+    FSS_TRY(T.posnode.ensure(sizeof(int32_t) * std::max<int64_t>(1, T.p0)));
+    hipLaunchKernelGGL(k_plan_posnode, dim3(nn), dim3(256), 0, s, T.counts.as<int32_t>(),
+                       T.offsets.as<int32_t>(), nn, T.posnode.as<int32_t>());
     FSS_TRY(T.inv_active.ensure(sizeof(int32_t) * (size_t)L * n));
     FSS_HIP(hipMemsetAsync(T.inv_active.p, 0xff, sizeof(int32_t) * (size_t)L * n, s));
     hipLaunchKernelGGL(k_plan_inverse, dim3(nn), dim3(256), 0, s, nch.as<int32_t>(),

@@ -81,6 +81,8 @@ SIGNATURES = {
     "fss_rk4_run_graph": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, _int, _int]),
     "fss_project_points": (_int, [_vp, _vp, _vp, _vp, _vp, _i64]),
     "fss_last_timing": (_int, [_vp, C.POINTER(_dbl), C.POINTER(_dbl)]),
+    "fss_timing_reset": (_int, [_vp]),
+    "fss_timing_mean": (_int, [_vp, C.POINTER(_i64), C.POINTER(_dbl), C.POINTER(_dbl)]),
     "fss_grid_icosahedral": (_int, [_int, _vp, _vp, _vp, _vp]),
     "fss_grid_voronoi_areas": (_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp]),
     "fss_grid_third_areas": (_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp]),
@@ -229,10 +231,13 @@ class Context:
         return Particles(_dev_ptr(x).value, _dev_ptr(y).value, _dev_ptr(z).value,
                          _dev_ptr(vort).value, _dev_ptr(area).value, x.numel())
 
-    def _out(self, m):
+    def _out(self, m, out=None):
         import torch
-        o = [torch.empty(m, dtype=torch.float64, device=self.device) for _ in range(4)]
-        return o, Velocity(*(t.data_ptr() for t in o))
+        if out is None:
+            out = [torch.empty(m, dtype=torch.float64, device=self.device) for _ in range(4)]
+        for t in out:
+            assert t.numel() == m and t.dtype == torch.float64 and t.is_contiguous()
+        return out, Velocity(*(t.data_ptr() for t in out))
 
     def tree_build(self, x, y, z):
         _check(lib().fss_tree_build(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), x.numel()))
@@ -264,14 +269,14 @@ class Context:
                     inter=inter[:7 * s.n_interactions].reshape(-1, 7),
                     tri_info=tri_info.reshape(T_all, 4), tri_xyz=tri_xyz.reshape(T_all, 3, 3))
 
-    def velocity_direct(self, x, y, z, vort, area):
-        o, ov = self._out(x.numel())
+    def velocity_direct(self, x, y, z, vort, area, out=None):
+        o, ov = self._out(x.numel(), out)
         p = self._particles(x, y, z, vort, area)
         _check(lib().fss_velocity_direct(self.h, C.byref(p), C.byref(ov)))
         return tuple(o)
 
-    def velocity_tree(self, x, y, z, vort, area):
-        o, ov = self._out(x.numel())
+    def velocity_tree(self, x, y, z, vort, area, out=None):
+        o, ov = self._out(x.numel(), out)
         p = self._particles(x, y, z, vort, area)
         _check(lib().fss_velocity_tree(self.h, C.byref(p), C.byref(ov)))
         return tuple(o)
@@ -301,3 +306,12 @@ class Context:
         a, b = C.c_double(), C.c_double()
         _check(lib().fss_last_timing(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def timing_reset(self):
+        _check(lib().fss_timing_reset(self.h))
+
+    def timing_mean(self):
+        """(calls, mean stage ms, mean main-kernel ms) since timing_reset()."""
+        n, a, b = C.c_int64(), C.c_double(), C.c_double()
+        _check(lib().fss_timing_mean(self.h, C.byref(n), C.byref(a), C.byref(b)))
+        return n.value, a.value, b.value

@@ -168,6 +168,11 @@ fss_status fss_project_points(fss_ctx *ctx, double *x, double *y, double *z, dou
 /* ---- timing of the last velocity call's kernels (HIP events on the
  * context stream): total stage ms and the dominant kernel's ms. ---- */
 fss_status fss_last_timing(fss_ctx *ctx, double *stage_ms, double *main_kernel_ms);
+/* Events are recorded into a ring of 256 slots without synchronizing;
+ * fss_timing_reset() starts a new window and fss_timing_mean() returns the
+ * mean over the calls since (at most 256). */
+fss_status fss_timing_reset(fss_ctx *ctx);
+fss_status fss_timing_mean(fss_ctx *ctx, int64_t *calls, double *stage_ms, double *main_kernel_ms);
 
 /* ---- grids and fields (host functions, host buffers) ----
  * Icosahedral particle grid after `level` refinements: V = 10*4^level + 2

@@ -163,3 +163,45 @@ def test_rk4_graph_replays_match_eager(fsslib, case4):
     ctx.rk4_run_graph(b[0], b[1], b[2], b[3], b[4], dt, 3, use_tree=True)
     for g, r in zip(_host(*b[:4]), _host(*a[:4])):
         np.testing.assert_array_equal(g, r)
+
+
+@pytest.fixture(scope="module")
+def random_case(fsslib):
+    n = 12000
+    x, y, z = fsslib.grid_random(n, 5)
+    area = np.full(n, 4 * np.pi / n)
+    vort = fsslib.gauss_vortices(x, y, z, area, 5, 2.0, 0.05, 0.3, seed=9)
+    return x, y, z, vort, area
+
+
+@pytest.mark.parametrize("L,evalm,rule,degree,theta", [(5, 0, 0, 2, 0.7), (6, 0, 0, 2, 0.5),
+                                                       (5, 1, 1, 2, 0.7), (5, 1, 1, 3, 0.6),
+                                                       (4, 1, 1, 5, 0.5)])
+def test_tree_random_points_all_types(fsslib, orc, random_case, L, evalm, rule, degree, theta):
+    """Uniform random points: uneven node counts, so pp, pc, cp and cc all occur."""
+    x, y, z, vort, area = random_case
+    print(f"random N={len(x)} L={L} eval={evalm} degree={degree} theta={theta}", flush=True)
+    ctx = fsslib.Context(tree_levels=L, assign_mode=0, eval_mode=evalm, node_rule=rule,
+                         degree=degree, theta=theta)
+    d = _dev(x, y, z, vort, area)
+    ctx.tree_build(*d[:3])
+    g = ctx.tree_export()
+    T = orc.Tree(x, y, z, L, theta=theta, many_count=10, degree=degree, assign=0, eval_mode=evalm,
+                 node_rule=rule)
+    inter = T.interactions()
+    np.testing.assert_array_equal(g["inter"], inter)
+    types = np.bincount(inter[:, 6], minlength=4)
+    print("types pp/pc/cp/cc", types)
+    got = _host(*ctx.velocity_tree(*d))
+    ref = T.rhs(vort, area, nthreads=8)
+    eu, ew = _vel_err(got, ref)
+    dref = orc.rhs_direct(x, y, z, vort, area)
+    print("rel-L2 vs oracle tree", eu, " tree vs direct", _vel_err(ref, dref)[0])
+    assert eu <= TOL_FP64 and ew <= TOL_FP64
+    c32 = fsslib.Context(tree_levels=L, assign_mode=0, eval_mode=evalm, node_rule=rule,
+                         degree=degree, theta=theta, precision=fsslib.MIXED)
+    c32.tree_build(*d[:3])
+    got32 = _host(*c32.velocity_tree(*d))
+    e32 = _vel_err(got32, got)[0]
+    print("mixed vs fp64", e32)
+    assert e32 <= TOL_MIXED

@@ -1,0 +1,417 @@
+"""CPU checks that pin the oracle (oracle/fss_oracle.c) without trusting it.
+
+The reference ships no golden vectors or tests for this path and cannot be
+compiled here (every header includes the macOS-only <Accelerate/Accelerate.h>),
+so each oracle function is pinned by what the mathematics fixes: brute force
+in long double, exact identities and invariants, closed-form solutions,
+independent geometric tests and exact-coverage checks.  A plausible mistake
+(a dropped term, a wrong sign, a transposed operand, an off-by-one level) fails
+at least one of them.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import icos_case, rel_l2
+
+
+# ---------------------------------------------------------------- LU / dgesv
+
+def test_lu_matches_numpy_solve(orc):
+    rng = np.random.default_rng(1)
+    for n in (3, 6, 10, 21):
+        for _ in range(20):
+            a = rng.normal(size=(n, n))
+            b = rng.normal(size=n)
+            lu, ipiv, info = orc.lu_factor(a)
+            assert info == 0
+            x = orc.lu_solve(lu, ipiv, b)
+            np.testing.assert_allclose(x, np.linalg.solve(a, b), rtol=1e-10, atol=1e-12)
+
+
+def test_lu_pivot_is_first_max(orc):
+    # column 0 has its largest magnitude in rows 1 and 2 (tie): idamax takes row 1
+    a = np.array([[1.0, 2.0, 3.0], [-4.0, 1.0, 0.0], [4.0, 0.0, 1.0]])
+    lu, ipiv, info = orc.lu_factor(a)
+    assert info == 0 and ipiv[0] == 1
+    # exact on an integer system with an exact solution
+    x = orc.lu_solve(lu, ipiv, a @ np.array([1.0, -2.0, 3.0]))
+    np.testing.assert_allclose(x, [1.0, -2.0, 3.0], rtol=0, atol=1e-14)
+
+
+def test_barycoords_singular_returns_rhs(orc):
+    # dgesv skips the solve when dgetrf reports a zero pivot: coords = p
+    p1 = np.array([1.0, 0.0, 0.0])
+    c = orc.barycoords(p1, p1, np.array([0.0, 1.0, 0.0]), np.array([0.3, 0.2, 0.1]))
+    np.testing.assert_array_equal(c, [0.3, 0.2, 0.1])
+
+
+def test_barycoords_reconstruct_point(orc):
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        V = rng.normal(size=(3, 3))
+        p = rng.normal(size=3)
+        c = orc.barycoords(V[0], V[1], V[2], p)
+        np.testing.assert_allclose(V.T @ c, p, rtol=1e-11, atol=1e-12)
+
+
+# ---------------------------------------------------------------- icosahedral tree
+
+def _sph_area(a, b, c):
+    trip = np.einsum("ij,ij->i", a, np.cross(b, c))
+    den = 1 + np.einsum("ij,ij->i", a, b) + np.einsum("ij,ij->i", b, c) + np.einsum("ij,ij->i", c, a)
+    return 2 * np.arctan2(np.abs(trip), den)
+
+
+def test_icos_counts_norms_and_partition(orc):
+    L = 5
+    v, tv, ti = orc.icos_init(L)
+    assert v.shape[0] == 10 * 4 ** L + 2
+    np.testing.assert_allclose(np.linalg.norm(v, axis=1), 1.0, rtol=0, atol=4e-16)
+    assert len(np.unique(v, axis=0)) == v.shape[0]  # dedupe by exact equality
+    tot = 4 * np.pi
+    for l in range(L + 1):
+        lo, hi = orc.level_offset(l), orc.level_offset(l + 1)
+        assert hi - lo == 20 * 4 ** l
+        t = tv[lo:hi]
+        A = _sph_area(v[t[:, 0]], v[t[:, 1]], v[t[:, 2]])
+        assert abs(A.sum() - tot) < 1e-11  # faces tile the sphere
+        # every vertex used on level l is one of the first 10*4^l+2
+        assert t.max() < 10 * 4 ** l + 2
+        if l < L:
+            ch = tv[hi:orc.level_offset(l + 2)].reshape(-1, 4, 3)
+            Ac = _sph_area(v[ch[..., 0].ravel()], v[ch[..., 1].ravel()], v[ch[..., 2].ravel()])
+            np.testing.assert_allclose(Ac.reshape(-1, 4).sum(1), A, rtol=1e-11)
+            # children share the parent's corners: child c keeps parent vertex (0, 2, 1)[c]
+            for c, pv in ((0, 0), (1, 2), (2, 1)):
+                np.testing.assert_array_equal(ch[:, c, 0], t[:, pv])
+
+
+def test_icos_centre_equidistant_and_radius(orc):
+    v, tv, ti = orc.icos_init(4)
+    P = v[tv]  # T x 3 x 3
+    c = ti[:, :3]
+    d = np.linalg.norm(P - c[:, None, :], axis=2)
+    np.testing.assert_allclose(d, np.broadcast_to(d[:, :1], d.shape), rtol=1e-9)  # circumcentre
+    np.testing.assert_allclose(ti[:, 3], d.max(1), rtol=1e-12)
+    np.testing.assert_allclose(np.linalg.norm(c, axis=1), 1.0, atol=4e-16)
+    # the centre lies inside its triangle: all barycentric coordinates > 0
+    coef = np.linalg.solve(np.transpose(P, (0, 2, 1)), c[:, :, None])[..., 0]
+    assert (coef > 0).all()
+
+
+def test_product_grid_equals_oracle_icos(orc, fsslib):
+    x, y, z, tris = fsslib.grid_icosahedral(4)
+    v, tv, _ = orc.icos_init(4)
+    np.testing.assert_array_equal(np.stack([x, y, z], 1), v)
+    np.testing.assert_array_equal(tris, tv[orc.level_offset(4):orc.level_offset(5)])
+
+
+# ---------------------------------------------------------------- point location
+
+@pytest.fixture(scope="module")
+def grid5(fsslib):
+    return icos_case(fsslib, 5)
+
+
+def test_robust_assignment_is_geometrically_right(orc, grid5):
+    x, y, z, vort, area, _ = grid5
+    L = 5
+    T = orc.Tree(x, y, z, L, assign=orc.ASSIGN_ROBUST)
+    loc, found = T.loc(), T.found()
+    assert found.all()
+    verts, tv = T.verts(), T.tri_verts()
+    p = np.stack([x, y, z], 1)
+    for l in range(L):
+        tri = tv[orc.level_offset(l) + loc[l]]
+        V = verts[tri]  # n x 3 x 3
+        c = np.linalg.solve(np.transpose(V, (0, 2, 1)), p[:, :, None])[..., 0]
+        assert c.min() > -1e-12  # inside its node (edge points to rounding)
+        if l > 0:
+            np.testing.assert_array_equal(loc[l] // 4, loc[l - 1])  # child of the previous node
+
+
+def test_parity_drops_only_edge_points(orc, grid5):
+    x, y, z, vort, area, _ = grid5
+    L = 6
+    T = orc.Tree(x, y, z, L, assign=orc.ASSIGN_PARITY)
+    R = orc.Tree(x, y, z, L, assign=orc.ASSIGN_ROBUST)
+    f, fr = T.found(), R.found()
+    assert fr.all()
+    drop = ~f
+    assert drop.any()  # the level-5 grid lies on tree edges: drops happen
+    verts, tv = R.verts(), R.tri_verts()
+    p = np.stack([x, y, z], 1)
+    first = drop.argmax(axis=0)
+    ids = np.nonzero(drop.any(axis=0))[0]
+    for i in ids[:500]:
+        l = first[i]
+        tri = tv[orc.level_offset(l) + R.loc()[l, i]]
+        c = np.linalg.solve(verts[tri].T, p[i])
+        assert abs(c.min()) < 1e-12  # a dropped particle lies on an edge of its true node
+
+
+def test_node_lists_are_sorted_and_consistent(orc, grid5):
+    x, y, z, vort, area, _ = grid5
+    T = orc.Tree(x, y, z, 4, assign=orc.ASSIGN_PARITY)
+    counts, offs, ids = T.counts(), T.offsets(), T.ids()
+    loc, found = T.loc(), T.found()
+    for l in range(4):
+        for k in range(0, 20 * 4 ** l, 7):
+            g = orc.level_offset(l) + k
+            lst = ids[offs[g]:offs[g] + counts[g]]
+            assert (np.diff(lst) > 0).all()
+            np.testing.assert_array_equal(lst, np.nonzero(found[l] & (loc[l] == k))[0])
+
+
+# ---------------------------------------------------------------- traversal
+
+def _mac(ti, lt, ls, it, is_, theta):
+    a = ti[orc_off(lt) + it]
+    b = ti[orc_off(ls) + is_]
+    cs = (a[:, :3] * b[:, :3]).sum(1) / (np.linalg.norm(a[:, :3], axis=1) * np.linalg.norm(b[:, :3], axis=1))
+    d = np.arccos(np.clip(cs, -1, 1))
+    with np.errstate(divide="ignore"):
+        sep = (a[:, 3] + b[:, 3]) / d
+    return d, sep
+
+
+def orc_off(l):
+    return 20 * (4 ** np.asarray(l, dtype=np.int64) - 1) // 3
+
+
+@pytest.mark.parametrize("assign", [0, 1])
+def test_traversal_generation_order_and_mac(orc, grid5, assign):
+    x, y, z, vort, area, _ = grid5
+    L, theta, mc = 5, 0.7, 10
+    T = orc.Tree(x, y, z, L, theta=theta, many_count=mc, assign=assign)
+    it = T.interactions()
+    lt, ls, t, s, nt, ns, ty = it.T
+    # FIFO order = generation order; a pair's generation is lev_t + lev_s
+    assert (np.diff(lt + ls) >= 0).all()
+    counts = T.counts()
+    np.testing.assert_array_equal(nt, counts[orc_off(lt) + t])
+    np.testing.assert_array_equal(ns, counts[orc_off(ls) + s])
+    assert (nt > 0).all() and (ns > 0).all()
+    d, sep = _mac(T.tri_info(), lt, ls, t, s, theta)
+    acc = (d > 0) & (sep < theta)
+    letters = (nt > mc) * 2 + (ns > mc)
+    np.testing.assert_array_equal(ty[acc], letters[acc])
+    near = ~acc
+    assert (ty[near] == 0).all()
+    ok = ((nt < mc) & (ns < mc)) | ((lt == L - 1) & (ls == L - 1))
+    assert ok[near].all()
+
+
+def test_traversal_covers_every_pair_once(orc, fsslib):
+    # small random set: every ordered particle pair (i, j) lies in exactly one
+    # interaction (target node holds i, source node holds j)
+    n = 400
+    x, y, z = fsslib.grid_random(n, 3)
+    T = orc.Tree(x, y, z, 4, theta=0.7, many_count=3, assign=orc.ASSIGN_ROBUST)
+    cover = np.zeros((n, n), dtype=np.int32)
+    for lt, ls, t, s, nt, ns, ty in T.interactions():
+        a = T.node_list(lt, t)
+        b = T.node_list(ls, s)
+        cover[np.ix_(a, b)] += 1
+    assert (cover == 1).all()
+
+
+def test_partition_invariant_robust(orc, grid5):
+    x, y, z, vort, area, _ = grid5
+    T = orc.Tree(x, y, z, 6, assign=orc.ASSIGN_ROBUST)
+    it = T.interactions()
+    assert int((it[:, 4].astype(np.int64) * it[:, 5]).sum()) == len(x) ** 2
+
+
+# ---------------------------------------------------------------- interpolation
+
+def test_degree2_reference_nodes_are_the_uniform_lattice(orc):
+    pts = orc.interp_nodes(2, 0)
+    want = np.array([[1, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, 1, 0], [0, 0, 1], [0, .5, .5]])
+    np.testing.assert_array_equal(pts, want)
+    # the intended rule gives the same set at degree 2
+    p1 = orc.interp_nodes(2, 1)
+    assert sorted(map(tuple, p1)) == sorted(map(tuple, want))
+
+
+def test_sin2_nodes_are_chebyshev_lobatto_on_edges(orc):
+    d = 4
+    pts = orc.interp_nodes(d, 1)
+    edge = pts[np.isclose(pts[:, 2], 0)]
+    cheb = (1 - np.cos(np.pi * np.arange(d + 1) / d)) / 2
+    np.testing.assert_allclose(np.sort(edge[:, 1]), cheb, atol=1e-15)
+
+
+@pytest.mark.parametrize("degree,rule", [(1, 0), (2, 0), (2, 1), (3, 1), (4, 1), (6, 1), (8, 1)])
+def test_interpolation_reproduces_polynomials(orc, degree, rule):
+    pts, lu, ipiv, info = orc.interp_init(degree, rule)
+    assert info == 0
+    rng = np.random.default_rng(degree)
+    p = pts.shape[0]
+    coef = rng.normal(size=p)
+    idx = [(i - j, j) for i in range(degree + 1) for j in range(i + 1)]
+    vals = np.array([sum(c * s ** a * t ** b for c, (a, b) in zip(coef, idx)) for s, t, _ in pts])
+    alpha = orc.lu_solve(lu, ipiv, vals)
+    np.testing.assert_allclose(alpha, coef, rtol=1e-9, atol=1e-9)
+
+
+# ---------------------------------------------------------------- direct sum
+
+def _brute_longdouble(x, y, z, vort, area):
+    P = np.stack([x, y, z], 1).astype(np.longdouble)
+    w = (vort * area).astype(np.longdouble)
+    n = len(x)
+    U = np.zeros((n, 3), dtype=np.longdouble)
+    for i in range(n):
+        d = 1 - P @ P[i]
+        cr = np.cross(P[i][None, :], P)
+        d[i] = 1
+        cr[i] = 0
+        U[i] = (cr * (w / d)[:, None]).sum(0)
+    return (U * (-1 / (4 * np.pi))).astype(np.float64)
+
+
+def test_direct_matches_long_double_brute_force(orc, fsslib):
+    x, y, z, vort, area, _ = icos_case(fsslib, 2)
+    u, v, w, dv = orc.rhs_direct(x, y, z, vort, area)
+    U = _brute_longdouble(x, y, z, vort, area)
+    assert rel_l2(np.stack([u, v, w], 1), U) < 1e-13
+    np.testing.assert_allclose(dv, -2 * 2 * np.pi * w, rtol=1e-15)
+    xr, yr, zr = fsslib.grid_random(300, 4)
+    ar = np.full(300, 4 * np.pi / 300)
+    vr = fsslib.gauss_vortices(xr, yr, zr, ar, 3, 2.0, 0.1, 0.3, seed=1)
+    u, v, w, dv = orc.rhs_direct(xr, yr, zr, vr, ar)
+    assert rel_l2(np.stack([u, v, w], 1), _brute_longdouble(xr, yr, zr, vr, ar)) < 1e-12
+
+
+def test_direct_invariants(orc, fsslib):
+    x, y, z, vort, area, _ = icos_case(fsslib, 4, field="gauss")
+    u, v, w, dv = orc.rhs_direct(x, y, z, vort, area)
+    U = np.stack([u, v, w], 1)
+    P = np.stack([x, y, z], 1)
+    # tangency
+    assert np.abs((U * P).sum(1)).max() < 1e-13 * np.abs(U).max()
+    # antisymmetry of K: sum_i w_i u_i = 0
+    wi = vort * area
+    assert np.linalg.norm((U * wi[:, None]).sum(0)) < 1e-12 * np.abs(U).max() * np.abs(wi).sum()
+    # rotation equivariance
+    th = 0.3
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    R = R @ np.array([[1, 0, 0], [0, np.cos(1.1), -np.sin(1.1)], [0, np.sin(1.1), np.cos(1.1)]])
+    Q = P @ R.T
+    u2, v2, w2, _ = orc.rhs_direct(Q[:, 0], Q[:, 1], Q[:, 2], vort, area)
+    np.testing.assert_allclose(np.stack([u2, v2, w2], 1), U @ R.T, atol=1e-12 * np.abs(U).max())
+
+
+def _rh_velocity(x, y, z, w0=0.5, K=0.5):
+    """Exact Rossby-Haurwitz velocity u = x cross grad psi with
+    psi = -w0 z + K z (x^4 - 6 x^2 y^2 + y^4) (so that lap psi = omega)."""
+    g = np.stack([K * z * (4 * x ** 3 - 12 * x * y ** 2),
+                  K * z * (-12 * x ** 2 * y + 4 * y ** 3),
+                  -w0 + K * (x ** 4 - 6 * x ** 2 * y ** 2 + y ** 4)], 1)
+    return np.cross(np.stack([x, y, z], 1), g)
+
+
+def test_product_rh_field_matches_cartesian_form(fsslib):
+    x, y, z, _ = fsslib.grid_icosahedral(3)
+    v, tr = fsslib.rossby_haurwitz(x, y, z, 0.5, 0.5)
+    want = 2 * 0.5 * z - 30 * 0.5 * z * (x ** 4 - 6 * x ** 2 * y ** 2 + y ** 4)
+    np.testing.assert_allclose(v, want, atol=1e-13)
+
+
+def test_direct_converges_to_closed_form(orc, fsslib):
+    errs = []
+    for level in (3, 4, 5):
+        x, y, z, vort, area, _ = icos_case(fsslib, level)
+        u, v, w, _ = orc.rhs_direct(x, y, z, vort, area)
+        errs.append(rel_l2(np.stack([u, v, w], 1), _rh_velocity(x, y, z)))
+    # measured 3.9e-2, 1.04e-2, 2.86e-3: second order in the grid spacing
+    assert errs[2] < 4e-3
+    assert errs[1] < errs[0] / 3 and errs[2] < errs[1] / 3
+
+
+def test_solid_body_rotation(orc, fsslib):
+    x, y, z, vort, area, _ = icos_case(fsslib, 5)
+    vort = 2 * 0.8 * z  # omega = 2 Omega' z  ->  u = Omega' z-hat x x
+    u, v, w, _ = orc.rhs_direct(x, y, z, vort, area)
+    want = 0.8 * np.stack([-y, x, np.zeros_like(z)], 1)
+    assert rel_l2(np.stack([u, v, w], 1), want) < 2e-3
+
+
+# ---------------------------------------------------------------- tree sum
+
+def test_tree_equals_direct_when_nothing_is_far(orc, grid5):
+    # theta -> 0: the MAC never accepts, every pair is evaluated directly
+    x, y, z, vort, area, _ = grid5
+    T = orc.Tree(x, y, z, 3, theta=1e-6, assign=orc.ASSIGN_ROBUST)
+    it = T.interactions()
+    assert (it[:, 6] == 0).all()
+    a = T.rhs(vort, area, nthreads=4)
+    b = orc.rhs_direct(x, y, z, vort, area)
+    assert rel_l2(np.stack(a[:3], 1), np.stack(b[:3], 1)) < 1e-13
+
+
+def test_bltc_converges_with_degree(orc, grid5):
+    x, y, z, vort, area, _ = grid5
+    ref = np.stack(orc.rhs_direct(x, y, z, vort, area)[:3], 1)
+    errs = []
+    for deg in (2, 4, 6):
+        T = orc.Tree(x, y, z, 5, theta=0.5, degree=deg, assign=1, eval_mode=orc.EVAL_BLTC,
+                     node_rule=1)
+        errs.append(rel_l2(np.stack(T.rhs(vort, area, nthreads=8)[:3], 1), ref))
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] < 1e-5
+
+
+def test_reference_cc_and_dispatch_variants_are_finite(orc, grid5):
+    x, y, z, vort, area, _ = grid5
+    ref = np.stack(orc.rhs_direct(x, y, z, vort, area)[:3], 1)
+    for mode in (orc.EVAL_PARITY, orc.EVAL_REFCC):
+        T = orc.Tree(x, y, z, 5, theta=0.7, assign=1, eval_mode=mode)
+        e = rel_l2(np.stack(T.rhs(vort, area, nthreads=8)[:3], 1), ref)
+        assert np.isfinite(e) and e < 0.2
+
+
+# ---------------------------------------------------------------- RK4 / projection
+
+def test_projection_conserves_absolute_vorticity(orc):
+    rng = np.random.default_rng(5)
+    p = rng.normal(size=(1000, 3))
+    p /= np.linalg.norm(p, axis=1)[:, None]
+    p *= (1 + 1e-3 * rng.normal(size=(1000, 1)))
+    vort = rng.normal(size=1000)
+    om = 2 * np.pi
+    x, y, z, v2 = orc.project_points(p[:, 0], p[:, 1], p[:, 2], vort, om)
+    np.testing.assert_allclose(np.sqrt(x ** 2 + y ** 2 + z ** 2), 1.0, atol=3e-16)
+    np.testing.assert_allclose(v2 + 2 * om * z, vort + 2 * om * p[:, 2], atol=1e-13)
+
+
+def test_rk4_is_fourth_order(orc, fsslib):
+    x, y, z, vort, area, _ = icos_case(fsslib, 2)
+    T_end = 0.08
+
+    def run(dt):
+        s = (x, y, z, vort)
+        for _ in range(int(round(T_end / dt))):
+            s = orc.rk4_step(*s, area, dt, nthreads=2)
+        return np.stack(s, 1)
+
+    ref = run(0.0025)
+    e1 = np.linalg.norm(run(0.04) - ref)
+    e2 = np.linalg.norm(run(0.02) - ref)
+    assert 10 < e1 / e2 < 22  # 2^4 = 16
+
+
+# ---------------------------------------------------------------- areas
+
+def test_areas_sum_to_4pi(orc, fsslib):
+    x, y, z, tris = fsslib.grid_icosahedral(4)
+    a = fsslib.voronoi_areas(x, y, z, tris)
+    b = fsslib.third_areas(x, y, z, tris)
+    c = orc.area_init(x, y, z, tris)
+    assert abs(a.sum() - 4 * np.pi) < 1e-12
+    assert abs(b.sum() - 4 * np.pi) < 1e-11
+    np.testing.assert_array_equal(b, c)
+    assert (a > 0).all()
