@@ -39,10 +39,10 @@ def log(*a):
 def workload(name):
     if name == "c2":
         return dict(name="c2", level=8, field="rh", tree_levels=6, degree=2, theta=0.7,
-                    many_count=10, targets=10000, seed=2024)
+                    many_count=10, targets=10000, seed=2024, headline="fp64")
     if name == "c3":
         return dict(name="c3", level=10, field="gauss", tree_levels=8, degree=3, theta=0.6,
-                    many_count=10, targets=10000, seed=3033)
+                    many_count=10, targets=10000, seed=3033, headline="mixed")
     raise SystemExit(f"unknown config {name}")
 
 
@@ -111,6 +111,17 @@ def cpu_baseline(W, x, y, z, vort, area):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     cores = min(16, os.cpu_count() or 1)
+    if len(x) > 2_000_000:
+        # the reference tree stage at this N is far beyond a bounded sample:
+        # time the reference direct sum (rhs_eval.h:15-32) on 400 targets
+        import numpy as np
+        idx = np.arange(0, len(x), len(x) // 400)[:400].astype(np.int64)
+        t0 = time.perf_counter()
+        oracle.rhs_direct(x, y, z, vort, area, idx=idx, nthreads=cores)
+        t = time.perf_counter() - t0
+        return dict(value=len(idx) / t, unit="velocities/s", cores=cores, kind="port",
+                    sample=f"direct sum at {len(idx)} evenly spaced targets x N={len(x)} sources, "
+                           f"{t:.2f} s on {cores} threads", seconds=t)
     T = oracle.Tree(x, y, z, W["tree_levels"], theta=W["theta"], many_count=W["many_count"],
                     degree=W["degree"], assign=oracle.ASSIGN_PARITY, eval_mode=oracle.EVAL_PARITY)
     t0 = time.perf_counter()
@@ -130,7 +141,13 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-variants", action="store_true", help="only the headline stage")
+    ap.add_argument("--profile", action="store_true",
+                    help="for rocprofv3 counter runs: headline stages only, then one plain copy "
+                         "of 512 MiB (torch clone) whose FETCH_SIZE/WRITE_SIZE calibrate the counters")
     args = ap.parse_args()
+    if args.profile:
+        args.no_cpu = True
+        args.no_variants = True
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -177,31 +194,42 @@ def main():
                          n_interactions=st.n_interactions, pairs_near=st.pairs_near,
                          pairs_far=st.pairs_far), [o.clone() for o in out]
 
-    # ---- headline: FP64 BLTC ----
-    ctx, head, u64 = run_tree(fss.FP64, fss.EVAL_BLTC, fss.ASSIGN_ROBUST, fss.NODES_SIN2,
-                              "bltc fp64", args.steps)
+    # ---- headline: the config's BLTC stage (C2: FP64; C3: mixed precision) ----
+    hprec = fss.MIXED if W["headline"] == "mixed" else fss.FP64
+    ctx, head, uh = run_tree(hprec, fss.EVAL_BLTC, fss.ASSIGN_ROBUST, fss.NODES_SIN2,
+                             f"bltc {W['headline']}", args.steps)
     spairs = self_pairs(ctx)
     alg_pairs = head["pairs_near"] - spairs
-    achieved = alg_pairs * FLOPS_PAIR_FP64 / (head["near_kernel_ms"] * 1e-3) / 1e12
-    roofline = dict(kernel="k_near64", bound="valu", achieved=achieved, peak=PEAK_FP64_TFLOPS,
-                    unit="TFLOP/s", frac=achieved / PEAK_FP64_TFLOPS,
-                    traffic=traffic_from_profiles("k_near64", W["name"]),
-                    flops_per_launch=alg_pairs * FLOPS_PAIR_FP64,
-                    note=f"{alg_pairs} near pairs x {FLOPS_PAIR_FP64} flops / mean k_near64 duration "
+    flp = FLOPS_PAIR_FP64 if hprec == fss.FP64 else FLOPS_PAIR_FP32
+    peak = PEAK_FP64_TFLOPS if hprec == fss.FP64 else PEAK_FP32_TFLOPS
+    kname = "k_near<double,2>" if hprec == fss.FP64 else "k_near<float,2>"
+    achieved = alg_pairs * flp / (head["near_kernel_ms"] * 1e-3) / 1e12
+    roofline = dict(kernel=kname, bound="valu", achieved=achieved, peak=peak,
+                    unit="TFLOP/s", frac=achieved / peak,
+                    traffic=traffic_from_profiles("k_near64" if hprec == fss.FP64 else "k_near32",
+                                                  W["name"]),
+                    flops_per_launch=alg_pairs * flp,
+                    note=f"{alg_pairs} near pairs x {flp} flops / mean {kname} duration "
                          f"(HIP events on the context stream)")
     del ctx
 
     variants = {}
     if not args.no_variants:
-        c, v, _ = run_tree(fss.FP64, fss.EVAL_PARITY, fss.ASSIGN_PARITY, fss.NODES_REFERENCE,
-                           "parity fp64 (reference dispatch)", max(3, args.steps // 4))
-        variants["parity_fp64"] = v
-        del c
-        c, v, u32 = run_tree(fss.MIXED, fss.EVAL_BLTC, fss.ASSIGN_ROBUST, fss.NODES_SIN2,
-                             "bltc mixed", args.steps)
-        v["rel_l2_vs_fp64_bltc"] = rel_errors([o.cpu().numpy() for o in u32],
-                                              [o.cpu().numpy() for o in u64])[0]
-        variants["bltc_mixed"] = v
+        if W["name"] == "c2":
+            c, v, _ = run_tree(fss.FP64, fss.EVAL_PARITY, fss.ASSIGN_PARITY, fss.NODES_REFERENCE,
+                               "parity fp64 (reference dispatch)", max(3, args.steps // 4))
+            variants["parity_fp64"] = v
+            del c
+        other = fss.FP64 if hprec == fss.MIXED else fss.MIXED
+        oname = "bltc_fp64" if other == fss.FP64 else "bltc_mixed"
+        c, v, uo = run_tree(other, fss.EVAL_BLTC, fss.ASSIGN_ROBUST, fss.NODES_SIN2,
+                            oname.replace("_", " "), args.steps)
+        u64, u32 = (uo, uh) if other == fss.FP64 else (uh, uo)
+        v["rel_l2_mixed_vs_fp64_bltc"] = rel_errors([o.cpu().numpy() for o in u32],
+                                                    [o.cpu().numpy() for o in u64])[0]
+        v["speedup_mixed_over_fp64"] = (head["value"] / v["value"] if hprec == fss.MIXED
+                                        else v["value"] / head["value"])
+        variants[oname] = v
         del c
         for prec, label in ((fss.FP64, "direct_fp64"), (fss.MIXED, "direct_mixed")):
             c = fss.Context(precision=prec)
@@ -214,6 +242,14 @@ def main():
                                    max_rel_vs_fp64_direct=mx,
                                    tflops=N * (N - 1) * fl / (main_ms * 1e-3) / 1e12)
             del c
+
+    if args.profile:
+        buf = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        torch.cuda.synchronize()
+        cp = buf.clone()  # reads 512 MiB, writes 512 MiB
+        torch.cuda.synchronize()
+        log(f"calibration copy: {buf.numel() * 8} bytes read and written")
+        del buf, cp
 
     cpu = None
     if not args.no_cpu:
@@ -231,10 +267,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if W["headline"] == "fp64" else "f32 kernels / f64 reductions",
         "data": "synthetic",
         "config": {
-            "workload": (f"{W['name']}: icosahedral-tree BLTC FP64, level-{W['level']} icosahedral "
+            "workload": (f"{W['name']}: icosahedral-tree BLTC {W['headline']}, level-{W['level']} icosahedral "
                          f"vertices N={N}, {'Rossby-Haurwitz w0=K=0.5' if W['field'] == 'rh' else '12 Gaussian vortices'}, "
                          f"dual-Voronoi areas, {W['tree_levels']} tree levels, degree {W['degree']}, "
                          f"theta {W['theta']}, many_count {W['many_count']}"),

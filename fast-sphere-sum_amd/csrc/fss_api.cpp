@@ -333,6 +333,24 @@ fss_status fss_tree_build(fss_ctx *ctx, const double *x, const double *y, const 
     })
 }
 
+fss_status fss_batch_build(fss_ctx *ctx, int ninst, const int64_t *n, const int *tree_levels,
+                           const double *x, const double *y, const double *z)
+{
+    FSS_GUARD({
+        if (!ctx || !n || !x || !y || !z || ninst < 1) return set_error("bad argument"), FSS_ERR_INVALID;
+        if (tree_levels)
+            for (int k = 0; k < ninst; k++)
+                if (tree_levels[k] < 1 || tree_levels[k] > 13)
+                    return set_error("tree_levels must be in [1, 13]"), FSS_ERR_INVALID;
+        FSS_HIP(hipSetDevice(ctx->cfg.device));
+        if (ctx->graph_exec) {
+            (void)hipGraphExecDestroy(ctx->graph_exec);
+            ctx->graph_exec = nullptr;
+        }
+        return batch_build(ctx, ninst, n, tree_levels, x, y, z);
+    })
+}
+
 fss_status fss_tree_stats_get(fss_ctx *ctx, fss_tree_stats *out)
 {
     if (!ctx || !out) return set_error("null argument"), FSS_ERR_INVALID;
@@ -413,6 +431,77 @@ fss_status fss_last_timing(fss_ctx *ctx, double *stage_ms, double *main_ms)
     if (stage_ms) *stage_ms = a;
     if (main_ms) *main_ms = b;
     return FSS_OK;
+}
+
+fss_status fss_sal_set_series(fss_ctx *ctx, const double *c, int nmax, int64_t table_points, double A)
+{
+    FSS_GUARD({
+        if (!ctx || !c) return set_error("null argument"), FSS_ERR_INVALID;
+        return sal_set_series(ctx, c, nmax, table_points, A);
+    })
+}
+
+fss_status fss_sal_kernel(fss_ctx *ctx, int kind, const double *s, int64_t m, double *g)
+{
+    if (!ctx || !s || !g || m < 1 || (kind != 0 && kind != 1)) return set_error("bad argument"), FSS_ERR_INVALID;
+    return sal_kernel_eval(ctx, kind, s, m, g);
+}
+
+fss_status fss_sal_direct(fss_ctx *ctx, const double *x, const double *y, const double *z,
+                          const double *eta, const double *area, int64_t n, const int64_t *idx,
+                          int64_t m, int kind, double *out)
+{
+    FSS_GUARD({
+        if (!ctx || !x || !y || !z || !eta || !area || !out || n < 1 || (idx && m < 1) ||
+            (kind != 0 && kind != 1))
+            return set_error("bad argument"), FSS_ERR_INVALID;
+        return sal_direct(ctx, x, y, z, eta, area, n, idx, m, kind, out);
+    })
+}
+
+int fss_sal_mlp_nparam(void) { return sal_nparam(); }
+
+fss_status fss_sal_mlp_init(fss_ctx *ctx, uint64_t seed)
+{
+    if (!ctx) return set_error("null context"), FSS_ERR_INVALID;
+    return sal_mlp_init(ctx, seed);
+}
+
+fss_status fss_sal_mlp_train(fss_ctx *ctx, int steps, int64_t samples, double lr, double s_min,
+                             uint64_t seed, double *final_loss)
+{
+    FSS_GUARD({
+        if (!ctx) return set_error("null context"), FSS_ERR_INVALID;
+        return sal_mlp_train(ctx, steps, samples, lr, s_min, seed, final_loss);
+    })
+}
+
+fss_status fss_sal_mlp_params(fss_ctx *ctx, float *params, int set)
+{
+    if (!ctx || !params) return set_error("null argument"), FSS_ERR_INVALID;
+    return sal_mlp_params(ctx, params, set);
+}
+
+fss_status fss_sal_mlp_norm(fss_ctx *ctx, double *mu, double *sigma, int set)
+{
+    if (!ctx || !mu || !sigma) return set_error("null argument"), FSS_ERR_INVALID;
+    if (set) {
+        ctx->sal.mu = *mu;
+        ctx->sal.sigma = *sigma;
+    } else {
+        *mu = ctx->sal.mu;
+        *sigma = ctx->sal.sigma;
+    }
+    return FSS_OK;
+}
+
+fss_status fss_sal_mlp_grad(fss_ctx *ctx, const float *x, const float *y, int64_t m, float *grad,
+                            float *loss)
+{
+    FSS_GUARD({
+        if (!ctx || !x || !y || !grad || !loss || m < 1) return set_error("bad argument"), FSS_ERR_INVALID;
+        return sal_mlp_grad(ctx, x, y, m, grad, loss);
+    })
 }
 
 fss_status fss_timing_reset(fss_ctx *ctx)

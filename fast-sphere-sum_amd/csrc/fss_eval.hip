@@ -604,24 +604,35 @@ template <int D>
 __global__ void k_combine(const int32_t *__restrict__ inv_active, const int32_t *__restrict__ nodeof,
                           const int32_t *__restrict__ node_slot, const double *__restrict__ alpha,
                           const double *__restrict__ tri_xyz, const double *__restrict__ x,
-                          const double *__restrict__ y, const double *__restrict__ z, int L,
-                          int64_t n, int64_t p0, const double *__restrict__ out, int normalise,
-                          double scale, double omega, double *u, double *v, double *w,
-                          double *dvort)
+                          const double *__restrict__ y, const double *__restrict__ z,
+                          const Instance *__restrict__ inst, int ninst, int64_t n, int64_t p0,
+                          const double *__restrict__ out, int normalise, double scale, double omega,
+                          double *u, double *v, double *w, double *dvort)
 {
     constexpr int P = npts<D>();
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // instance of particle i: last k with pbase <= i
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (inst[mid].pbase <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const Instance I = inst[lo];
+    const int64_t li = i - I.pbase;
+    const int L = I.L;
     double pt[3] = {x[i], y[i], z[i]};
     double a0 = 0, a1 = 0, a2 = 0;
     for (int l = 0; l < L; l++) {
-        int32_t pos = inv_active[(int64_t)l * n + i];
+        int64_t at = I.loff + (int64_t)l * I.n + li;
+        int32_t pos = inv_active[at];
         if (pos >= 0) {
             a0 += out[pos];
             a1 += out[p0 + pos];
             a2 += out[2 * p0 + pos];
         }
-        int32_t g = nodeof[(int64_t)l * n + i];
+        int32_t g = nodeof[at];
         int32_t sl = g >= 0 ? node_slot[g] : -1;
         if (sl >= 0) {
             double c[3], m[P];
@@ -729,9 +740,9 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
                            ctx->d_lu.as<double>(), ctx->d_ipiv.as<int32_t>(), ctx->alpha.as<double>());
     hipLaunchKernelGGL((k_combine<D>), dim3((unsigned)div_up(T.n, 256)), dim3(256), 0, s,
                        T.inv_active.as<int32_t>(), T.nodeof.as<int32_t>(), T.node_slot.as<int32_t>(),
-                       ctx->alpha.as<double>(), T.tri_xyz.as<double>(), pp->x, pp->y, pp->z, T.L, T.n,
-                       p0, ctx->out3.as<double>(), bltc ? 1 : 0, scale, ctx->cfg.omega, out->u,
-                       out->v, out->w, out->dvort);
+                       ctx->alpha.as<double>(), T.tri_xyz.as<double>(), pp->x, pp->y, pp->z,
+                       T.inst.as<Instance>(), T.ninst, T.n, p0, ctx->out3.as<double>(), bltc ? 1 : 0,
+                       scale, ctx->cfg.omega, out->u, out->v, out->w, out->dvort);
     if (ev) FSS_HIP(hipEventRecord(ev[3], s));
     FSS_HIP(hipGetLastError());
     return FSS_OK;

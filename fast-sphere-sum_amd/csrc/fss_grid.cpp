@@ -331,6 +331,34 @@ fss_status fss_field_gauss_vortices(int64_t n, const double *x, const double *y,
     return FSS_OK;
 }
 
+fss_status fss_sal_love_numbers(int nmax, uint64_t seed, double *h, double *k)
+{
+    if (nmax < 1 || !h || !k) return set_error("bad arguments"), FSS_ERR_INVALID;
+    Rng r(seed);
+    h[0] = 0.0;
+    k[0] = 0.0;
+    for (int n = 1; n <= nmax; n++) {
+        double u = 0.5 + r.uniform(), v = 0.5 + r.uniform();
+        h[n] = -6.2 + 3.0 * u / (n + 1);
+        k[n] = (-3.1 + v / (n + 1)) / n;
+    }
+    return FSS_OK;
+}
+
+// alpha_n (2n+1)/(4 pi) of GFnewtest.ipynb:83-94 with the Cesaro weights of :133-144
+fss_status fss_sal_coefficients(int nmax, const double *h, const double *k, double rho_ratio,
+                                int cesaro, double *c)
+{
+    if (nmax < 1 || !h || !k || !c) return set_error("bad arguments"), FSS_ERR_INVALID;
+    c[0] = 0.0;
+    for (int n = 1; n <= nmax; n++) {
+        double a = 3.0 * rho_ratio * (1.0 + k[n] - h[n]) / (4.0 * M_PI);
+        if (cesaro) a *= 1.0 - (double)n / (double)(nmax + 1);
+        c[n] = a;
+    }
+    return FSS_OK;
+}
+
 fss_status fss_sample_targets(int64_t n, int64_t m, uint64_t seed, int64_t *idx)
 {
     if (n < 1 || m < 0 || m > n || !idx) return set_error("bad arguments"), FSS_ERR_INVALID;

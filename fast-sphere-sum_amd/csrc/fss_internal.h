@@ -107,6 +107,17 @@ struct ProxySlot {
 // Sources per proxy work item before a node's class-B list is split.
 #define FSS_PROXY_CHUNK 4096
 
+// One problem instance of a (possibly batched) tree: particles
+// [pbase, pbase + n) of the concatenated state, L levels, and the offset of its
+// [L][n] blocks in nodeof / inv_active.
+struct Instance {
+    int64_t pbase;
+    int64_t n;
+    int64_t loff;
+    int32_t L;
+    int32_t pad;
+};
+
 // Interpolation constants (host copy + device copy).
 struct Interp {
     int degree = 2, p = 6;
@@ -148,6 +159,9 @@ struct Tree {
     DevBuf ranges_a;    // Range [n_ra]
     DevBuf ranges_b;    // Range [n_rb]
     DevBuf posnode;     // int32 [p0]: node of every level-ordered position
+    DevBuf inst;        // Instance[ninst]
+    int ninst = 1;
+    bool batched = false;
     DevBuf charge_nodes; // int32 [n_charge] nodes whose proxies carry charges (BLTC)
     DevBuf proxy_xyz;   // double4 [nn * p] static proxy positions (w unused)
     int64_t pairs_near = 0, pairs_far = 0;
@@ -157,6 +171,19 @@ struct Tree {
 // {stage start, main kernel start, main kernel end, stage end}.  Recording
 // never synchronizes; the mean over the slots since the last reset is read
 // with fss_timing_mean().
+// SAL state: series coefficients c_n (n = 0..nmax, c_0 = 0), the tabulated
+// series on a uniform s-grid, and the learned kernel's parameters.
+struct Sal {
+    int nmax = 0;
+    int64_t M = 0;       // table points
+    double A = 0;        // coefficient of the analytic 1/(2s) - 1 part
+    double g0 = 0;       // series at theta = 0 (self term)
+    double mu = 0, sigma = 1; // normalisation of the learned remainder
+    DevBuf c, tab, P, Pm, Pv, w;
+    bool has_mlp = false;
+    int64_t adam_t = 0;
+};
+
 struct Timing {
     static constexpr int RING = 256;
     hipEvent_t ev[RING][4] = {};
@@ -182,6 +209,7 @@ struct fss_ctx {
     fss::Interp interp;
     fss::Tree tree;
     fss::Timing timing;
+    fss::Sal sal;
 
     // device constants
     fss::DevBuf d_lu, d_ipiv, d_vinv, d_pts;
@@ -232,6 +260,25 @@ fss_status rk4_step(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
 fss_status rk4_run_graph(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
                          const double *area, int64_t n, double dt, int use_tree, int steps);
 fss_status project_points(fss_ctx *ctx, double *x, double *y, double *z, double *vort, int64_t n);
+
+// ---- batched instances (fss_batch.cpp) ----
+fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *levels, const double *x,
+                       const double *y, const double *z);
+
+// ---- SAL (fss_sal.hip) ----
+fss_status sal_set_series(fss_ctx *ctx, const double *c_host, int nmax, int64_t table_points, double A);
+fss_status sal_series_eval(fss_ctx *ctx, const double *s_dev, int64_t m, double *g_dev);
+fss_status sal_kernel_eval(fss_ctx *ctx, int kind, const double *s_dev, int64_t m, double *g_dev);
+fss_status sal_mlp_params(fss_ctx *ctx, float *host, int set);
+fss_status sal_mlp_init(fss_ctx *ctx, uint64_t seed);
+fss_status sal_mlp_train(fss_ctx *ctx, int steps, int64_t nsamp, double lr, double s_min,
+                         uint64_t seed, double *final_loss);
+fss_status sal_mlp_grad(fss_ctx *ctx, const float *x_host, const float *y_host, int64_t m,
+                        float *grad_host, float *loss_host);
+fss_status sal_direct(fss_ctx *ctx, const double *x, const double *y, const double *z,
+                      const double *eta, const double *area, int64_t n, const int64_t *idx,
+                      int64_t m, int kind, double *out);
+int sal_nparam();
 
 inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

@@ -767,6 +767,13 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
                                    hipMemcpyHostToDevice, s));
         FSS_HIP(hipStreamSynchronize(s));
     }
+    {
+        Instance one{0, n, 0, L, 0};
+        FSS_TRY(T.inst.ensure(sizeof(Instance)));
+        FSS_HIP(hipMemcpyAsync(T.inst.p, &one, sizeof(Instance), hipMemcpyHostToDevice, s));
+        T.ninst = 1;
+        T.batched = false;
+    }
     FSS_HIP(hipGetLastError());
     FSS_HIP(hipStreamSynchronize(s));
     T.built = true;
@@ -780,6 +787,10 @@ fss_status tree_export(fss_ctx *ctx, int32_t *loc, uint8_t *found, int32_t *coun
     hipStream_t s = ctx->stream;
     if (!T.built) {
         set_error("tree_export before tree_build");
+        return FSS_ERR_STATE;
+    }
+    if (T.batched) {
+        set_error("tree_export is not available for a batch");
         return FSS_ERR_STATE;
     }
     if (loc) FSS_TRY(d2h(loc, T.loc.p, sizeof(int32_t) * T.L * T.n, s));

@@ -72,6 +72,7 @@ SIGNATURES = {
     "fss_sync": (_int, [_vp]),
     "fss_tree_build": (_int, [_vp, _vp, _vp, _vp, _i64]),
     "fss_tree_stats_get": (_int, [_vp, C.POINTER(TreeStats)]),
+    "fss_batch_build": (_int, [_vp, _int, _vp, _vp, _vp, _vp, _vp]),
     "fss_tree_export": (_int, [_vp] + [_vp] * 8),
     "fss_velocity_direct": (_int, [_vp, C.POINTER(Particles), C.POINTER(Velocity)]),
     "fss_velocity_tree": (_int, [_vp, C.POINTER(Particles), C.POINTER(Velocity)]),
@@ -91,6 +92,17 @@ SIGNATURES = {
     "fss_field_gauss_vortices": (_int, [_i64, _vp, _vp, _vp, _vp, _int, _dbl, _dbl, _dbl, _u64,
                                         _vp]),
     "fss_sample_targets": (_int, [_i64, _i64, _u64, _vp]),
+    "fss_sal_love_numbers": (_int, [_int, _u64, _vp, _vp]),
+    "fss_sal_coefficients": (_int, [_int, _vp, _vp, _dbl, _int, _vp]),
+    "fss_sal_set_series": (_int, [_vp, _vp, _int, _i64, _dbl]),
+    "fss_sal_kernel": (_int, [_vp, _int, _vp, _i64, _vp]),
+    "fss_sal_direct": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _int, _vp]),
+    "fss_sal_mlp_nparam": (_int, []),
+    "fss_sal_mlp_init": (_int, [_vp, _u64]),
+    "fss_sal_mlp_train": (_int, [_vp, _int, _i64, _dbl, _dbl, _u64, C.POINTER(_dbl)]),
+    "fss_sal_mlp_params": (_int, [_vp, _vp, _int]),
+    "fss_sal_mlp_norm": (_int, [_vp, C.POINTER(_dbl), C.POINTER(_dbl), _int]),
+    "fss_sal_mlp_grad": (_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
 }
 
 _LIB = None
@@ -189,6 +201,28 @@ def sample_targets(n, m, seed):
     return idx
 
 
+def sal_love_numbers(nmax, seed):
+    """Synthetic load Love numbers h'_n, k'_n, n = 0..nmax (entry 0 unused)."""
+    h = np.zeros(nmax + 1)
+    k = np.zeros(nmax + 1)
+    _check(lib().fss_sal_love_numbers(nmax, seed, _np_ptr(h), _np_ptr(k)))
+    return h, k
+
+
+def sal_coefficients(h, k, rho_ratio=1 / 5.5, cesaro=False):
+    """c_n = 3 rho (1 + k'_n - h'_n)/(4 pi rho_e) (GFnewtest.ipynb:83-94), n = 0..nmax."""
+    nmax = len(h) - 1
+    c = np.zeros(nmax + 1)
+    _check(lib().fss_sal_coefficients(nmax, _np_ptr(np.ascontiguousarray(h)),
+                                      _np_ptr(np.ascontiguousarray(k)), rho_ratio, int(cesaro),
+                                      _np_ptr(c)))
+    return c
+
+
+def sal_mlp_nparam():
+    return lib().fss_sal_mlp_nparam()
+
+
 # ---------------- device context ----------------
 
 class Context:
@@ -241,6 +275,16 @@ class Context:
 
     def tree_build(self, x, y, z):
         _check(lib().fss_tree_build(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), x.numel()))
+        self.n = x.numel()
+
+    def batch_build(self, x, y, z, sizes, tree_levels=None):
+        """Trees for len(sizes) independent instances whose particles are
+        concatenated in x, y, z; later calls act on the concatenated state."""
+        n = np.ascontiguousarray(sizes, dtype=np.int64)
+        assert int(n.sum()) == x.numel()
+        lv = None if tree_levels is None else np.ascontiguousarray(tree_levels, dtype=np.int32)
+        _check(lib().fss_batch_build(self.h, len(n), _np_ptr(n), None if lv is None else _np_ptr(lv),
+                                     _dev_ptr(x), _dev_ptr(y), _dev_ptr(z)))
         self.n = x.numel()
 
     def stats(self):
@@ -306,6 +350,62 @@ class Context:
         a, b = C.c_double(), C.c_double()
         _check(lib().fss_last_timing(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    # ---- SAL ----
+    def sal_set_series(self, c, table_points=1 << 22, A=0.0):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        _check(lib().fss_sal_set_series(self.h, _np_ptr(c), len(c) - 1, table_points, A))
+
+    def sal_kernel(self, s, kind=0):
+        import torch
+        s = s.to(device=self.device, dtype=torch.float64).contiguous()
+        g = torch.empty_like(s)
+        _check(lib().fss_sal_kernel(self.h, kind, _dev_ptr(s), s.numel(), _dev_ptr(g)))
+        return g
+
+    def sal_direct(self, x, y, z, eta, area, idx=None, kind=0):
+        import torch
+        n = x.numel()
+        if idx is not None:
+            idx = idx.to(device=self.device, dtype=torch.int64).contiguous()
+            m, ip = idx.numel(), _dev_ptr(idx)
+        else:
+            m, ip = n, None
+        out = torch.empty(m, dtype=torch.float64, device=self.device)
+        _check(lib().fss_sal_direct(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), _dev_ptr(eta),
+                                    _dev_ptr(area), n, ip, m, kind, _dev_ptr(out)))
+        return out
+
+    def sal_mlp_init(self, seed=0):
+        _check(lib().fss_sal_mlp_init(self.h, seed))
+
+    def sal_mlp_train(self, steps, samples=1 << 16, lr=1e-3, s_min=1e-3, seed=0):
+        loss = C.c_double()
+        _check(lib().fss_sal_mlp_train(self.h, steps, samples, lr, s_min, seed, C.byref(loss)))
+        return loss.value
+
+    def sal_mlp_params(self, params=None):
+        n = sal_mlp_nparam()
+        if params is None:
+            p = np.zeros(n, dtype=np.float32)
+            _check(lib().fss_sal_mlp_params(self.h, _np_ptr(p), 0))
+            return p
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.shape == (n,)
+        _check(lib().fss_sal_mlp_params(self.h, _np_ptr(p), 1))
+
+    def sal_mlp_norm(self, mu=None, sigma=None):
+        a, b = C.c_double(mu or 0.0), C.c_double(sigma or 1.0)
+        _check(lib().fss_sal_mlp_norm(self.h, C.byref(a), C.byref(b), int(mu is not None)))
+        return a.value, b.value
+
+    def sal_mlp_grad(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(sal_mlp_nparam(), dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        _check(lib().fss_sal_mlp_grad(self.h, _np_ptr(x), _np_ptr(y), len(x), _np_ptr(g), _np_ptr(loss)))
+        return g, float(loss[0])
 
     def timing_reset(self):
         _check(lib().fss_timing_reset(self.h))

@@ -126,6 +126,16 @@ fss_status fss_tree_build(fss_ctx *ctx, const double *x, const double *y, const 
                           int64_t n);
 fss_status fss_tree_stats_get(fss_ctx *ctx, fss_tree_stats *out);
 
+/* Batched independent instances (no reference counterpart: the reference
+ * runs one problem per process).  x, y, z are the concatenation of ninst
+ * particle sets of sizes n[k] (device); tree_levels[k] per instance (NULL =
+ * the config's).  Afterwards fss_velocity_tree, fss_rk4_step and
+ * fss_rk4_run_graph act on the concatenated state (n = sum n[k]) with one
+ * launch per kernel for all instances; every instance only sees its own
+ * particles.  fss_tree_export is not available for a batch. */
+fss_status fss_batch_build(fss_ctx *ctx, int ninst, const int64_t *n, const int *tree_levels,
+                           const double *x, const double *y, const double *z);
+
 /* Copies integer tree data to HOST buffers for bit-exact checks (any pointer
  * may be NULL to skip it):
  *   loc      [tree_levels * n]  node of each particle per level (tree_point_locs)
@@ -200,6 +210,53 @@ fss_status fss_field_gauss_vortices(int64_t n, const double *x, const double *y,
                                     double rmax, uint64_t seed, double *vort);
 /* m distinct indices of [0, n) (partial Fisher-Yates) for error sampling. */
 fss_status fss_sample_targets(int64_t n, int64_t m, uint64_t seed, int64_t *idx);
+
+/* ---- self-attraction and loading (SAL) ----
+ * Green's function of SAL/GFnewtest.ipynb:83-131 (alpha, gfsphcoeff,
+ * greenfunc): G(theta) = sum_{n=1}^{nmax} c_n P_n(cos theta) with
+ * c_n = 3 rho_ratio (1 + k'_n - h'_n) / (4 pi), optionally Cesaro-weighted by
+ * 1 - n/(nmax+1) (cesgfsum, :133-144).  Separations are passed as
+ * s = sin(theta/2) = |x - y|/2.
+ *
+ * Synthetic load Love numbers (host): h'_n = -6.2 + 3 u_n/(n+1),
+ * n k'_n = -3.1 + v_n/(n+1), u_n, v_n ~ U(0.5, 1.5) seeded; arrays of
+ * nmax+1 entries indexed by n (entry 0 unused). */
+fss_status fss_sal_love_numbers(int nmax, uint64_t seed, double *h, double *k);
+/* c[0..nmax] (host), c[0] = 0. */
+fss_status fss_sal_coefficients(int nmax, const double *h, const double *k, double rho_ratio,
+                                int cesaro, double *c);
+/* Upload the series (host c) and tabulate it on table_points uniform s in
+ * [0, 1]; A = 3 rho_ratio (1 - h'_inf)/(4 pi) scales the analytic singular part
+ * A (1/(2s) - 1) of the learned kernel (sum_{n>=1} P_n = 1/(2 sin(theta/2)) - 1). */
+fss_status fss_sal_set_series(fss_ctx *ctx, const double *c, int nmax, int64_t table_points,
+                              double A);
+/* G at m separations s (device arrays): kind 0 = series (Clenshaw, FP64),
+ * kind 1 = learned kernel A (1/(2s) - 1) + mu + sigma * MLP(2s - 1). */
+fss_status fss_sal_kernel(fss_ctx *ctx, int kind, const double *s, int64_t m, double *g);
+/* salpotential (GFnewtest.ipynb:9934-9949): out[r] = sum_j G(d(i_r, j)) eta_j A_j
+ * including j = i (G at theta = 0 is the finite truncated series), for targets
+ * idx[0..m) (all n if idx == NULL).  Device arrays.  kind 0 = tabulated series
+ * (cubic interpolation), kind 1 = learned kernel. */
+fss_status fss_sal_direct(fss_ctx *ctx, const double *x, const double *y, const double *z,
+                          const double *eta, const double *area, int64_t n, const int64_t *idx,
+                          int64_t m, int kind, double *out);
+/* Learned kernel: MLP 1 -> 64 -> 64 -> 1 (tanh), FP32 parameters in the order
+ * W1[64], b1[64], W2[64][64] (row j = output unit), b2[64], w3[64], b3. */
+int fss_sal_mlp_nparam(void);
+fss_status fss_sal_mlp_init(fss_ctx *ctx, uint64_t seed);
+/* Adam on the normalised squared error of (G_series - A (1/(2s) - 1) - mu)/sigma
+ * over `samples` separations in [s_min, 1] (series values computed on the
+ * device); sets mu, sigma from the samples. */
+fss_status fss_sal_mlp_train(fss_ctx *ctx, int steps, int64_t samples, double lr, double s_min,
+                             uint64_t seed, double *final_loss);
+/* get (set = 0) or set (set = 1) the parameters (host array of nparam floats) */
+fss_status fss_sal_mlp_params(fss_ctx *ctx, float *params, int set);
+/* get or set the normalisation (mu, sigma) */
+fss_status fss_sal_mlp_norm(fss_ctx *ctx, double *mu, double *sigma, int set);
+/* mean-squared-error gradient on host inputs x (network input), y (target),
+ * m samples: grad[nparam] and loss (host) -- for checks against a CPU oracle */
+fss_status fss_sal_mlp_grad(fss_ctx *ctx, const float *x, const float *y, int64_t m, float *grad,
+                            float *loss);
 
 #ifdef __cplusplus
 }

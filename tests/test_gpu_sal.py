@@ -1,0 +1,100 @@
+"""GPU: SAL series kernel, SAL direct sum and the learned kernel (libfss,
+through the C ABI) against the numpy oracle (oracle/sal_oracle.py)."""
+import numpy as np
+import pytest
+
+import sal_oracle as S
+from conftest import icos_case
+
+pytestmark = pytest.mark.gpu
+
+NMAX = 4000
+
+
+@pytest.fixture(scope="module")
+def sal(fsslib):
+    import torch
+    h, k = fsslib.sal_love_numbers(NMAX, 5)
+    c = fsslib.sal_coefficients(h, k, 1 / 5.5, False)
+    A = 3 * (1 / 5.5) * (1 + 6.2) / (4 * np.pi)
+    ctx = fsslib.Context()
+    ctx.sal_set_series(c, table_points=1 << 22, A=A)
+    return ctx, c, A, torch
+
+
+def test_series_kernel_matches_oracle(sal):
+    ctx, c, A, torch = sal
+    rng = np.random.default_rng(2)
+    s = np.concatenate([rng.uniform(1e-3, 1, 3000), np.geomspace(1e-4, 1e-1, 200)])
+    print(f"series N={NMAX} at {len(s)} separations", flush=True)
+    g = ctx.sal_kernel(torch.from_numpy(s), kind=0).cpu().numpy()
+    want = S.green(c, s)
+    err = np.abs(g - want) / np.maximum(np.abs(want), 1e-3)
+    print("max rel err", err.max())
+    assert err.max() < 1e-11
+
+
+def test_sal_direct_table_matches_oracle(fsslib, sal):
+    ctx, c, A, torch = sal
+    x, y, z, _, area, _ = icos_case(fsslib, 5)
+    rng = np.random.default_rng(3)
+    eta = np.sin(3 * x) * np.cos(2 * z) + 0.1 * rng.normal(size=len(x))
+    idx = fsslib.sample_targets(len(x), 300, 7)
+    d = [torch.from_numpy(a).cuda() for a in (x, y, z, eta, area)]
+    print(f"sal direct N={len(x)} targets=300", flush=True)
+    got = ctx.sal_direct(*d, idx=torch.from_numpy(idx), kind=0).cpu().numpy()
+    want = S.salpotential(x, y, z, eta, area, lambda s: S.green(c, s), idx)
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print("rel-L2", err)
+    assert err < 1e-9
+
+
+def test_mlp_gradient_matches_oracle(fsslib, sal):
+    ctx, c, A, torch = sal
+    rng = np.random.default_rng(4)
+    p = (rng.normal(size=S.NPARAM) * 0.2).astype(np.float32)
+    ctx.sal_mlp_params(p)
+    np.testing.assert_array_equal(ctx.sal_mlp_params(), p)
+    x = rng.uniform(-1, 1, 1000).astype(np.float32)
+    y = rng.normal(size=1000).astype(np.float32)
+    g, loss = ctx.sal_mlp_grad(x, y)
+    l0, g0 = S.mlp_loss_grad(p.astype(np.float64), x.astype(np.float64), y.astype(np.float64))
+    print("loss", loss, l0, "grad rel", np.linalg.norm(g - g0) / np.linalg.norm(g0))
+    assert abs(loss - l0) < 1e-4 * abs(l0)
+    assert np.linalg.norm(g - g0) / np.linalg.norm(g0) < 1e-4
+
+
+def test_learned_kernel_trains_and_gains(fsslib, sal):
+    """The network trained on the device reduces the kernel error and the SAL
+    error against the series kernel below what the analytic singular part
+    alone gives (the method without the network)."""
+    ctx, c, A, torch = sal
+    ctx.sal_mlp_init(1)
+    l0 = ctx.sal_mlp_train(1, samples=1 << 15, lr=1e-3, s_min=1e-3, seed=2)
+    loss = ctx.sal_mlp_train(3000, samples=1 << 15, lr=1e-3, s_min=1e-3, seed=2)
+    print("loss first/last", l0, loss, flush=True)
+    assert loss < 0.05 * l0
+    s = np.geomspace(2e-3, 1, 500)
+    ts = torch.from_numpy(s)
+    g_ser = ctx.sal_kernel(ts, kind=0).cpu().numpy()
+    g_ml = ctx.sal_kernel(ts, kind=1).cpu().numpy()
+    mu, sigma = ctx.sal_mlp_norm()
+    g_sing = S.singular_part(s, A) + mu
+    e_ml = np.linalg.norm(g_ml - g_ser) / np.linalg.norm(g_ser)
+    e_sing = np.linalg.norm(g_sing - g_ser) / np.linalg.norm(g_ser)
+    print("kernel rel-L2 learned", e_ml, "singular part only", e_sing)
+    assert e_ml < 0.2 * e_sing
+    # the convolution
+    x, y, z, _, area, _ = icos_case(fsslib, 5)
+    eta = np.exp(-((x - 0.5) ** 2 + (y - 0.3) ** 2 + (z - 0.8) ** 2) / 0.05)
+    idx = fsslib.sample_targets(len(x), 500, 8)
+    d = [torch.from_numpy(a).cuda() for a in (x, y, z, eta, area)]
+    ref = ctx.sal_direct(*d, idx=torch.from_numpy(idx), kind=0).cpu().numpy()
+    got = ctx.sal_direct(*d, idx=torch.from_numpy(idx), kind=1).cpu().numpy()
+    e_conv = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    zero = np.zeros(S.NPARAM, dtype=np.float32)
+    ctx.sal_mlp_params(zero)
+    sing = ctx.sal_direct(*d, idx=torch.from_numpy(idx), kind=1).cpu().numpy()
+    e_conv_sing = np.linalg.norm(sing - ref) / np.linalg.norm(ref)
+    print("SAL rel-L2 learned", e_conv, "singular part only", e_conv_sing)
+    assert e_conv < e_conv_sing
