@@ -482,15 +482,18 @@ fss_status fss_sal_mlp_params(fss_ctx *ctx, float *params, int set)
     return sal_mlp_params(ctx, params, set);
 }
 
-fss_status fss_sal_mlp_norm(fss_ctx *ctx, double *mu, double *sigma, int set)
+fss_status fss_sal_mlp_norm(fss_ctx *ctx, double *mu, double *sigma, double *s_min, int set)
 {
-    if (!ctx || !mu || !sigma) return set_error("null argument"), FSS_ERR_INVALID;
+    if (!ctx || !mu || !sigma || !s_min) return set_error("null argument"), FSS_ERR_INVALID;
     if (set) {
+        if (!(*s_min > 0 && *s_min < 1) || !(*sigma > 0)) return set_error("sal: need 0 < s_min < 1, sigma > 0"), FSS_ERR_INVALID;
         ctx->sal.mu = *mu;
         ctx->sal.sigma = *sigma;
+        ctx->sal.s_min = *s_min;
     } else {
         *mu = ctx->sal.mu;
         *sigma = ctx->sal.sigma;
+        *s_min = ctx->sal.s_min;
     }
     return FSS_OK;
 }

@@ -58,29 +58,6 @@ struct Rng {
     }
 };
 
-struct Key {
-    uint64_t a, b, c;
-    bool operator==(const Key &o) const { return a == o.a && b == o.b && c == o.c; }
-};
-struct KeyHash {
-    size_t operator()(const Key &k) const
-    {
-        uint64_t h = k.a * 0x9e3779b97f4a7c15ull;
-        h ^= k.b + 0x7f4a7c159e3779b9ull + (h << 6) + (h >> 2);
-        h ^= k.c + 0x94d049bb133111ebull + (h << 6) + (h >> 2);
-        return (size_t)h;
-    }
-};
-
-Key key_of(const double *v)
-{
-    Key k;
-    std::memcpy(&k.a, v, 8);
-    std::memcpy(&k.b, v + 1, 8);
-    std::memcpy(&k.c, v + 2, 8);
-    return k;
-}
-
 // signed area of the spherical triangle (a, b, c) (Van Oosterom & Strackee)
 double sph_area_signed(const double *a, const double *b, const double *c)
 {

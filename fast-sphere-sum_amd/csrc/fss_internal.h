@@ -179,6 +179,7 @@ struct Sal {
     double A = 0;        // coefficient of the analytic 1/(2s) - 1 part
     double g0 = 0;       // series at theta = 0 (self term)
     double mu = 0, sigma = 1; // normalisation of the learned remainder
+    double s_min = 1e-3;      // the network input maps log s from [s_min, 1] onto [-1, 1]
     DevBuf c, tab, P, Pm, Pv, w;
     bool has_mlp = false;
     int64_t adam_t = 0;

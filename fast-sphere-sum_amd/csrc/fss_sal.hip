@@ -162,25 +162,35 @@ __global__ void k_adam(float *P, const float *__restrict__ g, float *mom, float 
     P[q] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
 }
 
+// Network input: s mapped logarithmically from [s_min, 1] onto [-1, 1]
+// (the remainder varies on the scale of s itself near small separations).
+__device__ inline float mlp_input(double s, float log_smin, float inv_span)
+{
+    float x = 2.f * (logf((float)s) - log_smin) * inv_span - 1.f;
+    return fmaxf(x, -1.5f);
+}
+
 __global__ void k_mlp_eval(const float *__restrict__ P, const double *__restrict__ s, int64_t m,
-                           double A, double mu, double sigma, double *out)
+                           double A, double mu, double sigma, float log_smin, float inv_span,
+                           double *out)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     float h1[H], h2[H];
     double si = s[i];
-    float y = mlp_forward(P, (float)(2.0 * si - 1.0), h1, h2);
+    float y = mlp_forward(P, mlp_input(si, log_smin, inv_span), h1, h2);
     out[i] = A * (0.5 / si - 1.0) + mu + sigma * (double)y;
 }
 
-// training inputs: x = 2 s - 1 (FP32), targets (G - singular - mu) / sigma
+// training inputs (FP32) and targets (G - singular - mu) / sigma
 __global__ void k_mlp_targets(const double *__restrict__ s, const double *__restrict__ g, int64_t m,
-                              double A, double mu, double sigma, float *xin, float *yt)
+                              double A, double mu, double sigma, float log_smin, float inv_span,
+                              float *xin, float *yt)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     double si = s[i];
-    xin[i] = (float)(2.0 * si - 1.0);
+    xin[i] = mlp_input(si, log_smin, inv_span);
     yt[i] = (float)((g[i] - A * (0.5 / si - 1.0) - mu) / sigma);
 }
 
@@ -198,7 +208,7 @@ __global__ void __launch_bounds__(STILE) k_sal_direct(const double *__restrict__
                                                       const double *__restrict__ tab, int64_t M,
                                                       const float *__restrict__ P, double A,
                                                       double mu, double sigma, double g0,
-                                                      double *out)
+                                                      float log_smin, float inv_span, double *out)
 {
     __shared__ double4 tile[STILE];
     __shared__ float Ps[NPARAM];
@@ -225,7 +235,7 @@ __global__ void __launch_bounds__(STILE) k_sal_direct(const double *__restrict__
                 gk = table_eval(tab, M, s);
             } else {
                 float h1[H], h2[H];
-                float yv = mlp_forward(Ps, (float)(2.0 * s - 1.0), h1, h2);
+                float yv = mlp_forward(Ps, mlp_input(s, log_smin, inv_span), h1, h2);
                 gk = A * (0.5 / s - 1.0) + mu + sigma * (double)yv;
             }
             acc = fma(gk, sj.w, acc);
@@ -354,9 +364,11 @@ fss_status sal_mlp_train(fss_ctx *ctx, int steps, int64_t nsamp, double lr, doub
     mean /= nsamp;
     double var = sq / nsamp - mean * mean;
     S.mu = mean;
+    S.s_min = s_min;
+    float lmin = (float)std::log(s_min), ispan = (float)(1.0 / -std::log(s_min));
     S.sigma = var > 0 ? std::sqrt(var) : 1.0;
     hipLaunchKernelGGL(k_mlp_targets, dim3((unsigned)div_up(nsamp, 256)), dim3(256), 0, s, ds.as<double>(),
-                       dg.as<double>(), nsamp, S.A, S.mu, S.sigma, dx.as<float>(), dy.as<float>());
+                       dg.as<double>(), nsamp, S.A, S.mu, S.sigma, lmin, ispan, dx.as<float>(), dy.as<float>());
     if (!S.Pm.p) {
         FSS_TRY(S.Pm.ensure(sizeof(float) * NPARAM));
         FSS_TRY(S.Pv.ensure(sizeof(float) * NPARAM));
@@ -417,7 +429,7 @@ fss_status sal_kernel_eval(fss_ctx *ctx, int kind, const double *s_dev, int64_t 
     if (kind == 0) return sal_series_eval(ctx, s_dev, m, g_dev);
     if (!S.has_mlp) return set_error("sal: no network"), FSS_ERR_STATE;
     hipLaunchKernelGGL(k_mlp_eval, dim3((unsigned)div_up(m, 256)), dim3(256), 0, s, S.P.as<float>(), s_dev,
-                       m, S.A, S.mu, S.sigma, g_dev);
+                       m, S.A, S.mu, S.sigma, (float)std::log(S.s_min), (float)(1.0 / -std::log(S.s_min)), g_dev);
     FSS_HIP(hipGetLastError());
     return FSS_OK;
 }
@@ -438,12 +450,13 @@ fss_status sal_direct(fss_ctx *ctx, const double *x, const double *y, const doub
         FSS_HIP(hipEventRecord(ctx->timing.slot()[1], s));
     }
     unsigned g = (unsigned)div_up(m, STILE);
+    float lmin = (float)std::log(S.s_min), ispan = (float)(1.0 / -std::log(S.s_min));
     if (kind == 0)
         hipLaunchKernelGGL((k_sal_direct<0>), dim3(g), dim3(STILE), 0, s, x, y, z, S.w.as<double>(), n, idx, m,
-                           S.tab.as<double>(), S.M, (const float *)nullptr, S.A, S.mu, S.sigma, S.g0, out);
+                           S.tab.as<double>(), S.M, (const float *)nullptr, S.A, S.mu, S.sigma, S.g0, lmin, ispan, out);
     else
         hipLaunchKernelGGL((k_sal_direct<1>), dim3(g), dim3(STILE), 0, s, x, y, z, S.w.as<double>(), n, idx, m,
-                           S.tab.as<double>(), S.M, S.P.as<float>(), S.A, S.mu, S.sigma, S.g0, out);
+                           S.tab.as<double>(), S.M, S.P.as<float>(), S.A, S.mu, S.sigma, S.g0, lmin, ispan, out);
     if (!ctx->capturing) {
         FSS_HIP(hipEventRecord(ctx->timing.slot()[2], s));
         FSS_HIP(hipEventRecord(ctx->timing.slot()[3], s));

@@ -101,7 +101,7 @@ SIGNATURES = {
     "fss_sal_mlp_init": (_int, [_vp, _u64]),
     "fss_sal_mlp_train": (_int, [_vp, _int, _i64, _dbl, _dbl, _u64, C.POINTER(_dbl)]),
     "fss_sal_mlp_params": (_int, [_vp, _vp, _int]),
-    "fss_sal_mlp_norm": (_int, [_vp, C.POINTER(_dbl), C.POINTER(_dbl), _int]),
+    "fss_sal_mlp_norm": (_int, [_vp, C.POINTER(_dbl), C.POINTER(_dbl), C.POINTER(_dbl), _int]),
     "fss_sal_mlp_grad": (_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
 }
 
@@ -394,10 +394,12 @@ class Context:
         assert p.shape == (n,)
         _check(lib().fss_sal_mlp_params(self.h, _np_ptr(p), 1))
 
-    def sal_mlp_norm(self, mu=None, sigma=None):
-        a, b = C.c_double(mu or 0.0), C.c_double(sigma or 1.0)
-        _check(lib().fss_sal_mlp_norm(self.h, C.byref(a), C.byref(b), int(mu is not None)))
-        return a.value, b.value
+    def sal_mlp_norm(self, mu=None, sigma=None, s_min=None):
+        """Get (no arguments) or set the learned remainder's normalisation
+        (mu, sigma) and the input range's lower end s_min."""
+        a, b, c = C.c_double(mu or 0.0), C.c_double(sigma or 1.0), C.c_double(s_min or 1e-3)
+        _check(lib().fss_sal_mlp_norm(self.h, C.byref(a), C.byref(b), C.byref(c), int(mu is not None)))
+        return a.value, b.value, c.value
 
     def sal_mlp_grad(self, x, y):
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -251,8 +251,9 @@ fss_status fss_sal_mlp_train(fss_ctx *ctx, int steps, int64_t samples, double lr
                              uint64_t seed, double *final_loss);
 /* get (set = 0) or set (set = 1) the parameters (host array of nparam floats) */
 fss_status fss_sal_mlp_params(fss_ctx *ctx, float *params, int set);
-/* get or set the normalisation (mu, sigma) */
-fss_status fss_sal_mlp_norm(fss_ctx *ctx, double *mu, double *sigma, int set);
+/* get or set the normalisation: remainder = mu + sigma * net(x) with the
+ * input x = 2 ln(s / s_min) / ln(1 / s_min) - 1 */
+fss_status fss_sal_mlp_norm(fss_ctx *ctx, double *mu, double *sigma, double *s_min, int set);
 /* mean-squared-error gradient on host inputs x (network input), y (target),
  * m samples: grad[nparam] and loss (host) -- for checks against a CPU oracle */
 fss_status fss_sal_mlp_grad(fss_ctx *ctx, const float *x, const float *y, int64_t m, float *grad,
