@@ -43,6 +43,9 @@ def workload(name):
     if name == "c3":
         return dict(name="c3", level=10, field="gauss", tree_levels=8, degree=3, theta=0.6,
                     many_count=10, targets=10000, seed=3033, headline="mixed")
+    if name == "c4":
+        return dict(name="c4", instances=64, nmin=2e4, nmax=1e6, degree=2, theta=0.7, many_count=10,
+                    targets=10000, seed=4044, headline="mixed", rk4_steps=5, dt=0.005)
     raise SystemExit(f"unknown config {name}")
 
 
@@ -133,6 +136,138 @@ def cpu_baseline(W, x, y, z, vort, area):
                 seconds=t)
 
 
+def run_batch(args, W, fss, torch):
+    """64 independent instances of uneven N (log-uniform in [2e4, 1e6]),
+    uniform random points with equal areas, 1-6 Gaussian vortices each; one
+    flattened plan (one launch per kernel for all instances), depth per
+    instance from the leaf-size autotuner, 5 RK4 steps replayed from a
+    hipGraph.  value = particle velocities per second over the 20 stages."""
+    import numpy as np
+    rng = np.random.default_rng(W["seed"])
+    K = W["instances"]
+    sizes = np.exp(rng.uniform(np.log(W["nmin"]), np.log(W["nmax"]), K)).astype(np.int64)
+    nvort = rng.integers(1, 7, K)
+    parts = []
+    t0 = time.time()
+    for k in range(K):
+        x, y, z = fss.grid_random(int(sizes[k]), W["seed"] + 1 + k)
+        a = np.full(len(x), 4 * np.pi / len(x))
+        v = fss.gauss_vortices(x, y, z, a, int(nvort[k]), 2.0, 0.05, 0.3, seed=W["seed"] + 100 + k)
+        parts.append((x, y, z, v, a))
+    cat = [np.concatenate([p[c] for p in parts]) for c in range(5)]
+    NT = len(cat[0])
+    base = np.concatenate([[0], np.cumsum(sizes)])
+    log(f"batch inputs: {K} instances, N total {NT}, min {sizes.min()} max {sizes.max()} "
+        f"in {time.time() - t0:.1f}s")
+    dev = [torch.from_numpy(a).to("cuda:0") for a in cat]
+
+    def make_ctx(precision, levels=1):
+        return fss.Context(tree_levels=levels, theta=W["theta"], degree=W["degree"],
+                           many_count=W["many_count"], precision=precision, eval_mode=fss.EVAL_BLTC,
+                           assign_mode=fss.ASSIGN_ROBUST, node_rule=fss.NODES_SIN2)
+
+    # autotune the leaf size on the median instance, transfer it by occupancy
+    km = int(np.argsort(sizes)[K // 2])
+    nk = int(sizes[km])
+    sl = slice(int(base[km]), int(base[km + 1]))
+    tctx = make_ctx(fss.MIXED)
+    best, tms = tctx.autotune_levels(*[d[sl] for d in dev], 3, 7, reps=3)
+    occ = nk / (20 * 4 ** (best - 1))
+    levels = [fss.levels_for(int(n), occ) for n in sizes]
+    log(f"autotune on N={nk}: ms per depth {tms} -> {best} levels, leaf occupancy {occ:.1f}")
+    del tctx
+
+    ctxs = {}
+    for prec in (fss.MIXED, fss.FP64):
+        c = make_ctx(prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        c.batch_build(*dev[:3], sizes, levels)
+        torch.cuda.synchronize()
+        ctxs[prec] = (c, (time.perf_counter() - t0) * 1e3)
+    ctx, build_ms = ctxs[fss.MIXED]
+    out = [torch.empty(NT, dtype=torch.float64, device="cuda:0") for _ in range(4)]
+    t_stage, near_ms = time_stage(torch, ctx, lambda: ctx.velocity_tree(*dev, out=out), args.steps,
+                                  args.warmup)
+    um = [o.cpu().numpy() for o in out]
+    c64 = ctxs[fss.FP64][0]
+    t_stage64, _ = time_stage(torch, c64, lambda: c64.velocity_tree(*dev, out=out), max(3, args.steps // 4),
+                              1)
+    ud = [o.cpu().numpy() for o in out]
+    st = ctx.stats()
+    # FP64 direct sum of each instance at its share of the sampled targets
+    per = max(1, W["targets"] // K)
+    ref, got = [[] for _ in range(3)], [[] for _ in range(3)]
+    rctx = fss.Context()
+    for k in range(K):
+        idx = fss.sample_targets(int(sizes[k]), min(per, int(sizes[k])), W["seed"] + 200 + k)
+        s = slice(int(base[k]), int(base[k + 1]))
+        r = rctx.velocity_direct_targets(*[d[s] for d in dev], torch.from_numpy(idx))
+        for c in range(3):
+            ref[c].append(r[c].cpu().numpy())
+            got[c].append(um[c][s][idx])
+    l2, mx = rel_errors([np.concatenate(g) for g in got], [np.concatenate(r) for r in ref])
+    l2m = rel_errors(um, ud)[0]
+    # 5 RK4 steps from the captured graph (first call captures and runs 1 step)
+    state = [d.clone() for d in dev]
+    ctx.rk4_run_graph(*state, W["dt"], 1, use_tree=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.rk4_run_graph(*state, W["dt"], W["rk4_steps"], use_tree=True)
+    torch.cuda.synchronize()
+    t_rk = time.perf_counter() - t0
+    stages = 4 * W["rk4_steps"]
+    value = NT * stages / t_rk
+    log(f"batch mixed stage {t_stage * 1e3:.3f} ms (fp64 {t_stage64 * 1e3:.3f} ms), near kernel "
+        f"{near_ms:.3f} ms, build {build_ms:.0f} ms; rk4 {W['rk4_steps']} steps {t_rk * 1e3:.2f} ms "
+        f"-> {value:.3e} vel/s; rel-L2 vs direct {l2:.3e}, mixed vs fp64 {l2m:.3e}")
+    # every particle meets itself in exactly one near interaction (robust
+    # assignment): NT masked self pairs are not algorithmic work
+    alg_pairs = st.pairs_near - NT
+    achieved = alg_pairs * FLOPS_PAIR_FP32 / (near_ms * 1e-3) / 1e12
+    roofline = dict(kernel="k_near<float,2>", bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS,
+                    unit="TFLOP/s", frac=achieved / PEAK_FP32_TFLOPS,
+                    traffic=traffic_from_profiles("k_near32", "c4"), flops_per_launch=alg_pairs * FLOPS_PAIR_FP32,
+                    note=f"{alg_pairs} near pairs x {FLOPS_PAIR_FP32} flops / mean "
+                         "k_near<float,2> duration (HIP events on the context stream)")
+    cpu = None
+    if not args.no_cpu:
+        # the oracle's reference-dispatch tree stage on the median instance
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        cores = min(16, os.cpu_count() or 1)
+        x, y, z, v, a = parts[km]
+        T = oracle.Tree(x, y, z, levels[km], theta=W["theta"], many_count=W["many_count"],
+                        degree=W["degree"], assign=oracle.ASSIGN_PARITY, eval_mode=oracle.EVAL_PARITY)
+        t0 = time.perf_counter()
+        T.rhs(v, a, nthreads=cores)
+        t = time.perf_counter() - t0
+        cpu = dict(value=nk / t, unit="velocities/s", cores=cores, kind="port",
+                   sample=f"one tree stage of the median instance (N={nk}, {levels[km]} levels, "
+                          f"reference dispatch) {t:.2f} s on {cores} threads", seconds=t)
+        log(f"cpu baseline: {cpu['value']:.3e} vel/s ({cpu['sample']})")
+    line = {
+        "metric": METRIC, "value": value, "unit": "velocities/s", "n_gpus": 1,
+        "steps": W["rk4_steps"], "warmup": 1, "ms_per_step": t_rk * 1e3 / W["rk4_steps"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 kernels / f64 reductions", "data": "synthetic",
+        "config": {"workload": (f"c4: {K} batched BLTC instances, N log-uniform in [2e4, 1e6] "
+                                f"(total {NT}), uniform random points, equal areas, 1-6 Gaussian "
+                                f"vortices, degree {W['degree']}, theta {W['theta']}, mixed precision, "
+                                f"{W['rk4_steps']} RK4 steps dt={W['dt']} from a hipGraph"),
+                   "n_total": NT, "instances": K, "seed": W["seed"], "degree": W["degree"],
+                   "theta": W["theta"], "tuned_levels": best, "leaf_occupancy": occ,
+                   "levels_hist": {int(l): int(levels.count(l)) for l in sorted(set(levels))},
+                   "parallelism": "1 GPU"},
+        "accuracy": {"rel_l2_vs_fp64_direct": l2, "max_rel_vs_fp64_direct": mx,
+                     "rel_l2_mixed_vs_fp64_bltc": l2m, "targets": per * K},
+        "stage_ms_mixed": t_stage * 1e3, "stage_ms_fp64": t_stage64 * 1e3,
+        "speedup_mixed_over_fp64": t_stage64 / t_stage, "tree_build_ms": build_ms,
+        "autotune_ms": tms, "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +276,8 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-variants", action="store_true", help="only the headline stage")
+    ap.add_argument("--prec", choices=("fp64", "mixed"), default=None,
+                    help="override the workload's headline precision (for profiling)")
     ap.add_argument("--profile", action="store_true",
                     help="for rocprofv3 counter runs: headline stages only, then one plain copy "
                          "of 512 MiB (torch clone) whose FETCH_SIZE/WRITE_SIZE calibrate the counters")
@@ -159,6 +296,10 @@ def main():
     import fss
 
     W = workload(args.config)
+    if W["name"] == "c4":
+        return run_batch(args, W, fss, torch)
+    if args.prec:
+        W["headline"] = args.prec
     x, y, z, vort, area = make_inputs(fss, W)
     N = len(x)
     dev = [torch.from_numpy(a).to("cuda:0") for a in (x, y, z, vort, area)]
@@ -173,7 +314,12 @@ def main():
 
     # FP64 direct sum at the sampled targets: the accuracy reference
     ref_ctx = fss.Context()
+    ref_ctx.velocity_direct_targets(*dev, tidx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     ref = [t.cpu().numpy() for t in ref_ctx.velocity_direct_targets(*dev, tidx)]
+    ref_s = time.perf_counter() - t0
+    log(f"fp64 direct reference at {len(idx)} targets: {ref_s * 1e3:.1f} ms")
 
     def run_tree(precision, eval_mode, assign, rule, label, steps):
         ctx = tree_ctx(precision, eval_mode, assign, rule)
@@ -231,7 +377,16 @@ def main():
                                         else v["value"] / head["value"])
         variants[oname] = v
         del c
-        for prec, label in ((fss.FP64, "direct_fp64"), (fss.MIXED, "direct_mixed")):
+        if N > 2_000_000:
+            # a full direct stage at this N takes minutes: report the sampled
+            # FP64 direct sum (the accuracy reference) as a pair rate instead
+            pairs = len(idx) * (N - 1)
+            variants["direct_fp64_sampled"] = dict(
+                targets=len(idx), ms=ref_s * 1e3, value=len(idx) / ref_s,
+                tflops=pairs * FLOPS_PAIR_FP64 / ref_s / 1e12,
+                note="FP64 direct sum at the sampled targets, host-timed incl. the copy back")
+        for prec, label in (((fss.FP64, "direct_fp64"), (fss.MIXED, "direct_mixed"))
+                            if N <= 2_000_000 else ()):
             c = fss.Context(precision=prec)
             t, main_ms = time_stage(torch, c, lambda: c.velocity_direct(*dev, out=out), 2, 1)
             got = [o.cpu().numpy()[idx] for o in out]

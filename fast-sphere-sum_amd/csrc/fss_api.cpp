@@ -333,6 +333,23 @@ fss_status fss_tree_build(fss_ctx *ctx, const double *x, const double *y, const 
     })
 }
 
+fss_status fss_autotune_levels(fss_ctx *ctx, const fss_particles *p, int lmin, int lmax, int reps,
+                               int *best_levels, double *ms)
+{
+    FSS_GUARD({
+        if (!ctx || !p || !p->x || !p->y || !p->z || !p->vort || !p->area || p->n < 1)
+            return set_error("null argument"), FSS_ERR_INVALID;
+        FSS_HIP(hipSetDevice(ctx->cfg.device));
+        if (ctx->graph_exec) {
+            (void)hipGraphExecDestroy(ctx->graph_exec);
+            ctx->graph_exec = nullptr;
+        }
+        return autotune_levels(ctx, p, lmin, lmax, reps, best_levels, ms);
+    })
+}
+
+int fss_levels_for(int64_t n, double leaf_occupancy) { return levels_for(n, leaf_occupancy); }
+
 fss_status fss_batch_build(fss_ctx *ctx, int ninst, const int64_t *n, const int *tree_levels,
                            const double *x, const double *y, const double *z)
 {

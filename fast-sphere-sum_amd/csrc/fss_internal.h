@@ -262,6 +262,11 @@ fss_status rk4_run_graph(fss_ctx *ctx, double *x, double *y, double *z, double *
                          const double *area, int64_t n, double dt, int use_tree, int steps);
 fss_status project_points(fss_ctx *ctx, double *x, double *y, double *z, double *vort, int64_t n);
 
+// ---- leaf-size autotuner (fss_tune.cpp) ----
+int levels_for(int64_t n, double occupancy);
+fss_status autotune_levels(fss_ctx *ctx, const fss_particles *pp, int lmin, int lmax, int reps, int *best,
+                           double *ms);
+
 // ---- batched instances (fss_batch.cpp) ----
 fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *levels, const double *x,
                        const double *y, const double *z);

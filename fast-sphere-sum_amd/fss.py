@@ -73,6 +73,8 @@ SIGNATURES = {
     "fss_tree_build": (_int, [_vp, _vp, _vp, _vp, _i64]),
     "fss_tree_stats_get": (_int, [_vp, C.POINTER(TreeStats)]),
     "fss_batch_build": (_int, [_vp, _int, _vp, _vp, _vp, _vp, _vp]),
+    "fss_autotune_levels": (_int, [_vp, C.POINTER(Particles), _int, _int, _int, C.POINTER(_int), _vp]),
+    "fss_levels_for": (_int, [C.c_int64, _dbl]),
     "fss_tree_export": (_int, [_vp] + [_vp] * 8),
     "fss_velocity_direct": (_int, [_vp, C.POINTER(Particles), C.POINTER(Velocity)]),
     "fss_velocity_tree": (_int, [_vp, C.POINTER(Particles), C.POINTER(Velocity)]),
@@ -219,6 +221,11 @@ def sal_coefficients(h, k, rho_ratio=1 / 5.5, cesaro=False):
     return c
 
 
+def levels_for(n, leaf_occupancy):
+    """Tree depth whose leaves hold leaf_occupancy particles on average."""
+    return lib().fss_levels_for(int(n), float(leaf_occupancy))
+
+
 def sal_mlp_nparam():
     return lib().fss_sal_mlp_nparam()
 
@@ -286,6 +293,17 @@ class Context:
         _check(lib().fss_batch_build(self.h, len(n), _np_ptr(n), None if lv is None else _np_ptr(lv),
                                      _dev_ptr(x), _dev_ptr(y), _dev_ptr(z)))
         self.n = x.numel()
+
+    def autotune_levels(self, x, y, z, vort, area, lmin, lmax, reps=3):
+        """Pick the fastest tree depth in [lmin, lmax] on these particles;
+        the context keeps it.  Returns (best_levels, {levels: ms})."""
+        p = self._particles(x, y, z, vort, area)
+        ms = np.zeros(lmax - lmin + 1)
+        best = C.c_int(0)
+        _check(lib().fss_autotune_levels(self.h, C.byref(p), lmin, lmax, reps, C.byref(best), _np_ptr(ms)))
+        self.n = x.numel()
+        self.cfg.tree_levels = best.value
+        return best.value, {lmin + k: float(v) for k, v in enumerate(ms)}
 
     def stats(self):
         s = TreeStats()

@@ -136,6 +136,19 @@ fss_status fss_tree_stats_get(fss_ctx *ctx, fss_tree_stats *out);
 fss_status fss_batch_build(fss_ctx *ctx, int ninst, const int64_t *n, const int *tree_levels,
                            const double *x, const double *y, const double *z);
 
+/* Leaf-size autotuner (no reference counterpart: the reference takes the
+ * depth by hand, run_config.h -l).  Builds the tree for every tree_levels in
+ * [lmin, lmax] on the device particles p, times `reps` velocity stages after
+ * a warm-up (HIP events on the context stream) and keeps the fastest depth:
+ * afterwards the context's config and tree are those of *best_levels.
+ * ms (host, lmax - lmin + 1 entries, may be NULL) receives the mean stage
+ * time of every depth. */
+fss_status fss_autotune_levels(fss_ctx *ctx, const fss_particles *p, int lmin, int lmax, int reps,
+                               int *best_levels, double *ms);
+/* Depth whose leaves hold `leaf_occupancy` particles on average for n
+ * particles: round(1 + log4(n / (20 occupancy))), clamped to [1, 13]. */
+int fss_levels_for(int64_t n, double leaf_occupancy);
+
 /* Copies integer tree data to HOST buffers for bit-exact checks (any pointer
  * may be NULL to skip it):
  *   loc      [tree_levels * n]  node of each particle per level (tree_point_locs)

@@ -39,3 +39,30 @@ def test_batch_matches_single_instances(fsslib, precision, evalm):
     ctx.rk4_run_graph(state[0], state[1], state[2], state[3], state[4], 0.005, 2, use_tree=True)
     r = np.sqrt(sum(t.cpu().numpy() ** 2 for t in state[:3]))
     np.testing.assert_allclose(r, 1.0, atol=1e-15)
+
+
+def test_autotune_levels_keeps_fastest_depth(fsslib):
+    """The autotuner times every depth, keeps the fastest one in the context
+    (config and tree), and that tree gives exactly the result of a context
+    built at that depth directly."""
+    import torch
+    n = 30000
+    x, y, z = fsslib.grid_random(n, 7)
+    a = np.full(n, 4 * np.pi / n)
+    v = fsslib.gauss_vortices(x, y, z, a, 3, 2.0, 0.05, 0.3, seed=1)
+    d = [torch.from_numpy(t).cuda() for t in (x, y, z, v, a)]
+    print(f"autotune n={n} depths 2..5", flush=True)
+    ctx = fsslib.Context(precision=1, eval_mode=1, assign_mode=1, node_rule=1)
+    best, ms = ctx.autotune_levels(*d, 2, 5, reps=2)
+    assert sorted(ms) == [2, 3, 4, 5] and all(t > 0 for t in ms.values())
+    assert best == min(ms, key=ms.get) and ctx.cfg.tree_levels == best
+    assert ctx.stats().tree_levels == best
+    got = [t.cpu().numpy() for t in ctx.velocity_tree(*d)]
+    c1 = fsslib.Context(precision=1, eval_mode=1, assign_mode=1, node_rule=1, tree_levels=best)
+    c1.tree_build(*d[:3])
+    ref = [t.cpu().numpy() for t in c1.velocity_tree(*d)]
+    for c in range(4):
+        np.testing.assert_array_equal(got[c], ref[c])
+    occ = n / (20 * 4 ** (best - 1))
+    assert fsslib.levels_for(n, occ) == best
+    assert fsslib.levels_for(16 * n, occ) == min(13, best + 2)
