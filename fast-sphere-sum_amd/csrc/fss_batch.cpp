@@ -10,9 +10,10 @@
 // (elementwise on the concatenated state) and its hipGraph work unchanged.
 //
 // Merged source layout: the particle records of all instances first
-// (instance k at pbase_src[k]), then the proxies of all nodes of all
-// instances (global node g at P0 + g * p), which keeps the single-tree
-// addressing `p0 + node * p` of the kernels.
+// (instance k at sbase[k]), then the proxies of all nodes of all instances
+// (global node g at P0 + g * p), which keeps the single-tree addressing
+// `p0 + node * p` of the kernels.  Output slots of the near kernel are
+// offset the same way (obase[k]).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,8 +44,9 @@ template <class T> fss_status up(DevBuf &d, const std::vector<T> &h, hipStream_t
 
 struct Part {
     int L;
-    int64_t n, nn, p0, n_charge;
-    std::vector<int32_t> ids, posnode, counts, offsets, inv, nodeof, node_slot, charge;
+    int64_t n, nn, p0, n_charge, nslot_out;
+    std::vector<int32_t> recid, recnode, counts, inv, nodeof, node_slot, charge;
+    std::vector<int4> nrange;
     std::vector<double> tri_info, tri_xyz;
     std::vector<double4> proxy;
     std::vector<NearItem> near;
@@ -77,11 +79,12 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         P.nn = T.nn;
         P.p0 = T.p0;
         P.n_charge = T.n_charge;
+        P.nslot_out = T.nslot_out;
         P.stats = T.stats;
-        FSS_TRY(down(P.ids, T.ids, T.p0, s));
-        FSS_TRY(down(P.posnode, T.posnode, T.p0, s));
+        FSS_TRY(down(P.recid, T.recid, T.p0, s));
+        FSS_TRY(down(P.recnode, T.recnode, T.p0, s));
+        FSS_TRY(down(P.nrange, T.nrange, T.nn, s));
         FSS_TRY(down(P.counts, T.counts, T.nn, s));
-        FSS_TRY(down(P.offsets, T.offsets, T.nn, s));
         FSS_TRY(down(P.inv, T.inv_active, (int64_t)T.L * T.n, s));
         FSS_TRY(down(P.nodeof, T.nodeof, (int64_t)T.L * T.n, s));
         FSS_TRY(down(P.node_slot, T.node_slot, T.nn, s));
@@ -97,9 +100,9 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         pb += ns[k];
     }
     // offsets
-    int64_t P0 = 0, NN = 0, NA = 0, NB = 0, NS = 0, NPART = 0, LN = 0;
+    int64_t P0 = 0, NN = 0, NA = 0, NB = 0, NS = 0, NPART = 0, LN = 0, NO = 0;
     std::vector<int64_t> sbase(ninst), nbase(ninst), rabase(ninst), rbbase(ninst), slbase(ninst),
-        ptbase(ninst), lbase(ninst), pbase(ninst);
+        ptbase(ninst), lbase(ninst), pbase(ninst), obase(ninst);
     pb = 0;
     for (int k = 0; k < ninst; k++) {
         const Part &P = parts[k];
@@ -111,7 +114,9 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         slbase[k] = NS;
         ptbase[k] = NPART;
         lbase[k] = LN;
+        obase[k] = NO;
         pb += P.n;
+        NO += P.nslot_out;
         P0 += P.p0;
         NN += P.nn;
         NA += P.ra.size();
@@ -120,7 +125,7 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         NPART += P.prox.size();
         LN += (int64_t)P.L * P.n;
     }
-    if (P0 > INT32_MAX || NN * p + P0 > INT32_MAX || LN > INT32_MAX)
+    if (P0 > INT32_MAX || NN * p + P0 > INT32_MAX || LN > INT32_MAX || NO > INT32_MAX)
         return set_error("batch: merged plan exceeds 32-bit positions"), FSS_ERR_INVALID;
     auto remap_range = [&](Range r, int k) {
         int32_t c = r.count;
@@ -134,7 +139,8 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         r.node = (int32_t)(r.node + nbase[k]);
         return r;
     };
-    std::vector<int32_t> ids, posnode, counts, offsets, inv, nodeof, node_slot, charge;
+    std::vector<int32_t> recid, recnode, counts, inv, nodeof, node_slot, charge;
+    std::vector<int4> nrange;
     std::vector<double> tri_info, tri_xyz;
     std::vector<double4> proxy;
     std::vector<NearItem> near;
@@ -145,11 +151,15 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
     fss_tree_stats tot{};
     for (int k = 0; k < ninst; k++) {
         Part &P = parts[k];
-        for (int32_t v : P.ids) ids.push_back((int32_t)(v + pbase[k]));
-        for (int32_t v : P.posnode) posnode.push_back((int32_t)(v + nbase[k]));
+        for (int32_t v : P.recid) recid.push_back((int32_t)(v + pbase[k]));
+        for (int32_t v : P.recnode) recnode.push_back((int32_t)(v + nbase[k]));
+        for (int4 r : P.nrange) {
+            r.x = (int32_t)(r.x + sbase[k]);
+            r.z = (int32_t)(r.z + sbase[k]);
+            nrange.push_back(r);
+        }
         counts.insert(counts.end(), P.counts.begin(), P.counts.end());
-        for (int32_t v : P.offsets) offsets.push_back((int32_t)(v + sbase[k]));
-        for (int32_t v : P.inv) inv.push_back(v >= 0 ? (int32_t)(v + sbase[k]) : -1);
+        for (int32_t v : P.inv) inv.push_back(v >= 0 ? (int32_t)(v + obase[k]) : -1);
         for (int32_t v : P.nodeof) nodeof.push_back(v >= 0 ? (int32_t)(v + nbase[k]) : -1);
         for (int32_t v : P.node_slot) node_slot.push_back(v >= 0 ? (int32_t)(v + slbase[k]) : -1);
         for (int32_t v : P.charge) charge.push_back((int32_t)(v + nbase[k]));
@@ -158,6 +168,7 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         proxy.insert(proxy.end(), P.proxy.begin(), P.proxy.end());
         for (NearItem it : P.near) {
             it.tgt_start = (int32_t)(it.tgt_start + sbase[k]);
+            it.out_base = (int32_t)(it.out_base + obase[k]);
             it.a_beg = (int32_t)(it.a_beg + rabase[k]);
             it.a_end = (int32_t)(it.a_end + rabase[k]);
             it.node = (int32_t)(it.node + nbase[k]);
@@ -190,6 +201,8 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
     T.n = pb;
     T.nn = NN;
     T.p0 = P0;
+    T.nslot_out = NO;
+    T.nlist = 0;
     T.L = 0;
     for (const Part &P : parts) T.L = std::max(T.L, P.L);
     T.ntri_all = NN;
@@ -208,10 +221,10 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
     tot.tree_levels = T.L;
     tot.n_nodes = NN;
     T.stats = tot;
-    FSS_TRY(up(T.ids, ids, s));
-    FSS_TRY(up(T.posnode, posnode, s));
+    FSS_TRY(up(T.recid, recid, s));
+    FSS_TRY(up(T.recnode, recnode, s));
+    FSS_TRY(up(T.nrange, nrange, s));
     FSS_TRY(up(T.counts, counts, s));
-    FSS_TRY(up(T.offsets, offsets, s));
     FSS_TRY(up(T.inv_active, inv, s));
     FSS_TRY(up(T.nodeof, nodeof, s));
     FSS_TRY(up(T.node_slot, node_slot, s));
@@ -228,6 +241,8 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
     T.loc.release();
     T.found.release();
     T.inter.release();
+    T.ids.release();
+    T.offsets.release();
     FSS_HIP(hipStreamSynchronize(s));
     T.built = true;
     return FSS_OK;

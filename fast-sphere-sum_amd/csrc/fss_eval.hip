@@ -113,23 +113,23 @@ __device__ inline void pair_far(const float4 &xt, const float4 &s, float off, fl
     pair_eval(xt, s, off, S0, S1, S2);
 }
 
-// Shift that brings a range's records to the target node's frame: zero for
-// FP64 (absolute coordinates), centre(range node) - centre(target node) for
-// the FP32 offset records.
-__device__ inline double4 range_shift(const double4 *, const double *, int, int)
+// A record moved to the frame of a target node with centre ct: FP64 records
+// are absolute; an FP32 record (offset o from the centre of node rn) becomes
+// (c_rn - ct) + o, summed in FP64 and rounded once.
+__device__ inline double4 to_frame(const double4 &s, int32_t, const double *, const double *) { return s; }
+__device__ inline float4 to_frame(const float4 &s, int32_t rn, const double *__restrict__ centres,
+                                  const double *ct)
 {
-    return make_double4(0, 0, 0, 0);
+    const double *c = centres + 4 * (int64_t)rn;
+    return make_float4((float)((c[0] - ct[0]) + (double)s.x), (float)((c[1] - ct[1]) + (double)s.y),
+                       (float)((c[2] - ct[2]) + (double)s.z), s.w);
 }
-__device__ inline float4 range_shift(const float4 *, const double *__restrict__ centres, int src_node,
-                                     int tgt_node)
+
+// record node of source position j: its own for particle records, the
+// owning node for proxies (p0 + node * p)
+__device__ inline int32_t src_node(int64_t j, int64_t p0, int p, const int32_t *__restrict__ recnode)
 {
-    const double *a = centres + 4 * (int64_t)src_node, *b = centres + 4 * (int64_t)tgt_node;
-    return make_float4((float)(a[0] - b[0]), (float)(a[1] - b[1]), (float)(a[2] - b[2]), 0.0f);
-}
-__device__ inline double4 shifted(const double4 &s, const double4 &) { return s; }
-__device__ inline float4 shifted(const float4 &s, const float4 &d)
-{
-    return make_float4(s.x + d.x, s.y + d.y, s.z + d.z, s.w);
+    return j < p0 ? recnode[j] : (int32_t)((j - p0) / p);
 }
 
 // 3x3 solve in reference order (interpolation coordinates; not bit-exact work)
@@ -200,17 +200,17 @@ template <int D> __device__ inline void monomials(double s, double t, double *m)
 template <int D> constexpr int npts() { return (D + 1) * (D + 2) / 2; }
 
 // ------------------------------------------------------------------
-// gather: level-ordered sources
+// gather: source records in tree order, then the proxies
 // ------------------------------------------------------------------
 __global__ void k_gather64(const double *__restrict__ x, const double *__restrict__ y,
                            const double *__restrict__ z, const double *__restrict__ vort,
-                           const double *__restrict__ area, const int32_t *__restrict__ ids,
+                           const double *__restrict__ area, const int32_t *__restrict__ recid,
                            int64_t p0, const double4 *__restrict__ proxy, int64_t nprox,
                            double4 *src)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < p0) {
-        int32_t id = ids[i];
+        int32_t id = recid[i];
         src[i] = make_double4(x[id], y[id], z[id], vort[id] * area[id]);
     } else if (i < p0 + nprox) {
         double4 q = proxy[i - p0];
@@ -220,9 +220,9 @@ __global__ void k_gather64(const double *__restrict__ x, const double *__restric
 
 __global__ void k_gather32(const double *__restrict__ x, const double *__restrict__ y,
                            const double *__restrict__ z, const double *__restrict__ vort,
-                           const double *__restrict__ area, const int32_t *__restrict__ ids,
+                           const double *__restrict__ area, const int32_t *__restrict__ recid,
                            int64_t p0, const double4 *__restrict__ proxy, int64_t nprox, int p,
-                           const int32_t *__restrict__ posnode, const double *__restrict__ centres,
+                           const int32_t *__restrict__ recnode, const double *__restrict__ centres,
                            int planar, float4 *src, float *aoff)
 {
     // FP32 records hold the offset from the centre of the record's node
@@ -230,8 +230,8 @@ __global__ void k_gather32(const double *__restrict__ x, const double *__restric
     // ~1e-9 absolute accuracy instead of the 6e-8 of rounded coordinates.
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < p0) {
-        int32_t id = ids[i];
-        const double *c = centres + 4 * (int64_t)posnode[i];
+        int32_t id = recid[i];
+        const double *c = centres + 4 * (int64_t)recnode[i];
         src[i] = make_float4((float)(x[id] - c[0]), (float)(y[id] - c[1]), (float)(z[id] - c[2]),
                              (float)(vort[id] * area[id]));
     } else if (i < p0 + nprox) {
@@ -249,35 +249,49 @@ __global__ void k_gather32(const double *__restrict__ x, const double *__restric
 // upward pass (BLTC): modified charges q_k = sum_j L_k(b_j) w_j
 // one 256-thread block per charge node
 // ------------------------------------------------------------------
+// position (double) and weight of a particle record
+__device__ inline void rec_point(const double4 &r, int32_t, const double *, double *pt, double &w)
+{
+    pt[0] = r.x;
+    pt[1] = r.y;
+    pt[2] = r.z;
+    w = r.w;
+}
+__device__ inline void rec_point(const float4 &r, int32_t rn, const double *__restrict__ centres, double *pt,
+                                 double &w)
+{
+    const double *c = centres + 4 * (int64_t)rn;
+    pt[0] = c[0] + (double)r.x;
+    pt[1] = c[1] + (double)r.y;
+    pt[2] = c[2] + (double)r.z;
+    w = (double)r.w;
+}
+
 template <typename Real, int D>
 __global__ void __launch_bounds__(256) k_upward(const int32_t *__restrict__ charge_nodes,
-                                                const int32_t *__restrict__ offsets,
-                                                const int32_t *__restrict__ counts,
+                                                const int4 *__restrict__ nrange,
+                                                const int32_t *__restrict__ recnode,
+                                                const double *__restrict__ centres,
                                                 const double *__restrict__ tri_xyz,
-                                                const double *__restrict__ x,
-                                                const double *__restrict__ y,
-                                                const double *__restrict__ z,
-                                                const double *__restrict__ vort,
-                                                const double *__restrict__ area,
-                                                const int32_t *__restrict__ ids,
                                                 const double *__restrict__ vinv, int64_t p0,
-                                                Real *src4)
+                                                typename Rec<Real>::T4 *src)
 {
     constexpr int P = npts<D>();
     __shared__ double red[4][P];
     int32_t g = charge_nodes[blockIdx.x];
-    int32_t off = offsets[g], cnt = counts[g];
+    const int4 nr = nrange[g];
+    const int32_t cnt = nr.y + nr.w;
     const double *T = tri_xyz + 9 * (int64_t)g;
     double M[P];
 #pragma unroll
     for (int k = 0; k < P; k++) M[k] = 0.0;
     for (int32_t j = threadIdx.x; j < cnt; j += 256) {
-        int32_t id = ids[off + j];
-        double pt[3] = {x[id], y[id], z[id]}, c[3], m[P];
+        int32_t pos = j < nr.y ? nr.x + j : nr.z + (j - nr.y);
+        double pt[3], c[3], m[P], w;
+        rec_point(src[pos], recnode[pos], centres, pt, w);
         bary_solve(T, pt, c);
         double sg = c[0] + c[1] + c[2];
         monomials<D>(c[0] / sg, c[1] / sg, m);
-        double w = vort[id] * area[id];
 #pragma unroll
         for (int k = 0; k < P; k++) M[k] = fma(m[k], w, M[k]);
     }
@@ -297,7 +311,7 @@ __global__ void __launch_bounds__(256) k_upward(const int32_t *__restrict__ char
             q = fma(vinv[idx + k * P], Mi, q); // vinv column-major: Vinv[idx][k]
         }
         int64_t pos = p0 + (int64_t)g * P + k;
-        src4[4 * pos + 3] = (Real)q; // weight slot of the (x, y, z, w) record
+        src[pos].w = (Real)q; // weight slot of the proxy record
     }
 }
 
@@ -306,18 +320,17 @@ __global__ void __launch_bounds__(256) k_upward(const int32_t *__restrict__ char
 // ------------------------------------------------------------------
 // Stages the sources of ranges [r0, r1) that are not flagged self into LDS in
 // chunks and calls body(k) for every staged entry k owned by this lane.
-// Records are moved to the target node's frame (range_shift) while staged.
+// Records are moved to the target node's frame (to_frame) while staged.
 template <typename T4, typename Body>
 __device__ inline void staged_ranges(const Range *__restrict__ ranges, int r0, int r1,
-                                     const T4 *__restrict__ src, const double *__restrict__ centres,
-                                     int tgt_node, T4 *lds, int ph, int P, bool act, Body body,
-                                     int &fill)
+                                     const T4 *__restrict__ src, const int32_t *__restrict__ recnode,
+                                     int64_t p0, int p, const double *__restrict__ centres, const double *ct,
+                                     T4 *lds, int ph, int P, bool act, Body body, int &fill)
 {
     const int lane = threadIdx.x;
     for (int r = r0; r < r1; r++) {
         Range rg = ranges[r];
         if (rg.count & FSS_SELF_BIT) continue;
-        T4 sh = range_shift(src, centres, rg.node, tgt_node);
         for (int base = 0; base < rg.count; base += 64) {
             int m = min(64, rg.count - base);
             if (fill + m > NCH) {
@@ -328,7 +341,10 @@ __device__ inline void staged_ranges(const Range *__restrict__ ranges, int r0, i
                 __syncthreads();
                 fill = 0;
             }
-            if (lane < m) lds[fill + lane] = shifted(src[rg.start + base + lane], sh);
+            if (lane < m) {
+                int64_t j = (int64_t)rg.start + base + lane;
+                lds[fill + lane] = to_frame(src[j], src_node(j, p0, p, recnode), centres, ct);
+            }
             fill += m;
         }
     }
@@ -358,11 +374,12 @@ template <typename Real, int R>
 __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
                                              const Range *__restrict__ ranges,
                                              const typename Rec<Real>::T4 *__restrict__ src,
-                                             const int32_t *__restrict__ ids, int64_t p0,
+                                             const int32_t *__restrict__ recid,
+                                             const int32_t *__restrict__ recnode, int64_t p0, int p,
                                              const double *__restrict__ centres,
                                              const double *__restrict__ x,
                                              const double *__restrict__ y,
-                                             const double *__restrict__ z, double *out)
+                                             const double *__restrict__ z, int64_t ostride, double *out)
 {
     using T4 = typename Rec<Real>::T4;
     __shared__ T4 lds[NCH];
@@ -374,14 +391,17 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
     const int P = 64 / T;
     const int ph = lane / T, tl = lane - ph * T;
     const bool act = ph < P;
+    const double *cp = centres + 4 * (int64_t)it.node;
+    const double ct[3] = {cp[0], cp[1], cp[2]};
     T4 xt[R];
     int32_t tid[R];
 #pragma unroll
     for (int q = 0; q < R; q++) {
         int t = tl + q * T;
         int tt = (act && t < cnt) ? t : 0;
-        xt[q] = src[it.tgt_start + tt];
-        tid[q] = ids[it.tgt_start + tt];
+        int64_t j = (int64_t)it.tgt_start + tt;
+        xt[q] = to_frame(src[j], recnode[j], centres, ct);
+        tid[q] = recid[j];
     }
     double S[R][3];
     Real Pp[R][3];
@@ -408,21 +428,20 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
         for (int q = 0; q < R; q++) pair_eval(xt[q], s, (Real)0, Pp[q][0], Pp[q][1], Pp[q][2]);
     };
     int fill = 0;
-    staged_ranges(ranges, it.a_beg, it.a_end, src, centres, it.node, lds, ph, P, act, body, fill);
+    staged_ranges(ranges, it.a_beg, it.a_end, src, recnode, p0, p, centres, ct, lds, ph, P, act, body, fill);
     __syncthreads();
     if (act)
         for (int k = ph; k < fill; k += P) body(k);
     body(-1);
-    // ranges that hold the target itself: skip the self pair by particle id
+    // ranges that may hold the target itself: skip the self pair by particle id
     for (int r = it.a_beg; r < it.a_end; r++) {
         Range rg = ranges[r];
         if (!(rg.count & FSS_SELF_BIT)) continue;
         int c0 = rg.count & ~FSS_SELF_BIT;
-        T4 sh = range_shift(src, centres, rg.node, it.node);
         if (act)
             for (int j = rg.start + ph; j < rg.start + c0; j += P) {
-                T4 s = shifted(src[j], sh);
-                int32_t sid = ids[j];
+                T4 s = to_frame(src[j], recnode[j], centres, ct);
+                int32_t sid = recid[j];
 #pragma unroll
                 for (int q = 0; q < R; q++)
                     if (sid != tid[q]) pair_eval(xt[q], s, (Real)0, Pp[q][0], Pp[q][1], Pp[q][2]);
@@ -448,10 +467,10 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
             double xa[3];
             abs_pos(xt[q], tid[q], x, y, z, xa);
             double x0 = xa[0], x1 = xa[1], x2 = xa[2];
-            int64_t pos = it.tgt_start + t;
+            int64_t pos = it.out_base + t;
             out[pos] = x1 * a2 - x2 * a1;
-            out[p0 + pos] = x2 * a0 - x0 * a2;
-            out[2 * p0 + pos] = x0 * a1 - x1 * a0;
+            out[ostride + pos] = x2 * a0 - x0 * a2;
+            out[2 * ostride + pos] = x0 * a1 - x1 * a0;
         }
     }
 }
@@ -463,6 +482,7 @@ template <typename Real, int D>
 __global__ void __launch_bounds__(64) k_proxy(const ProxyItem *__restrict__ items,
                                               const Range *__restrict__ ranges,
                                               const typename Rec<Real>::T4 *__restrict__ src,
+                                              const int32_t *__restrict__ recnode,
                                               const float *__restrict__ aoff, int64_t p0,
                                               const double *__restrict__ centres,
                                               const double4 *__restrict__ proxy_xyz, double *phi)
@@ -512,8 +532,10 @@ __global__ void __launch_bounds__(64) k_proxy(const ProxyItem *__restrict__ item
 #pragma unroll
         for (int q = 0; q < R; q++) pair_far(yk[q], s, at[q], Pp[q][0], Pp[q][1], Pp[q][2]);
     };
+    const double *cp = centres + 4 * (int64_t)it.node;
+    const double ct[3] = {cp[0], cp[1], cp[2]};
     int fill = 0;
-    staged_ranges(ranges, it.b_beg, it.b_end, src, centres, it.node, lds, ph, P, act, body, fill);
+    staged_ranges(ranges, it.b_beg, it.b_end, src, recnode, p0, PP, centres, ct, lds, ph, P, act, body, fill);
     __syncthreads();
     if (act)
         for (int k = ph; k < fill; k += P) body(k);
@@ -605,7 +627,7 @@ __global__ void k_combine(const int32_t *__restrict__ inv_active, const int32_t 
                           const int32_t *__restrict__ node_slot, const double *__restrict__ alpha,
                           const double *__restrict__ tri_xyz, const double *__restrict__ x,
                           const double *__restrict__ y, const double *__restrict__ z,
-                          const Instance *__restrict__ inst, int ninst, int64_t n, int64_t p0,
+                          const Instance *__restrict__ inst, int ninst, int64_t n, int64_t ostride,
                           const double *__restrict__ out, int normalise, double scale, double omega,
                           double *u, double *v, double *w, double *dvort)
 {
@@ -629,8 +651,8 @@ __global__ void k_combine(const int32_t *__restrict__ inv_active, const int32_t 
         int32_t pos = inv_active[at];
         if (pos >= 0) {
             a0 += out[pos];
-            a1 += out[p0 + pos];
-            a2 += out[2 * p0 + pos];
+            a1 += out[ostride + pos];
+            a2 += out[2 * ostride + pos];
         }
         int32_t g = nodeof[at];
         int32_t sl = g >= 0 ? node_slot[g] : -1;
@@ -679,7 +701,8 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
     const bool bltc = ctx->cfg.eval_mode == FSS_EVAL_BLTC;
     const int64_t p0 = T.p0, nprox = T.nn * p, nsrc = p0 + nprox;
     const double scale = -1.0 / (4.0 * M_PI);
-    FSS_TRY(ctx->out3.ensure(sizeof(double) * 3 * std::max<int64_t>(1, p0)));
+    const int64_t ost = std::max<int64_t>(1, T.nslot_out);
+    FSS_TRY(ctx->out3.ensure(sizeof(double) * 3 * ost));
     FSS_TRY(ctx->phi.ensure(sizeof(double) * 3 * p * std::max<int64_t>(1, T.n_proxy)));
     FSS_TRY(ctx->alpha.ensure(sizeof(double) * 3 * p * std::max<int64_t>(1, T.n_slot)));
     if (ev) FSS_HIP(hipEventRecord(ev[0], s));
@@ -688,50 +711,48 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
         FSS_TRY(ctx->src64.ensure(sizeof(double4) * nsrc));
         double4 *src = ctx->src64.as<double4>();
         hipLaunchKernelGGL(k_gather64, dim3(gs), dim3(256), 0, s, pp->x, pp->y, pp->z, pp->vort,
-                           pp->area, T.ids.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, src);
+                           pp->area, T.recid.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, src);
         if (bltc && T.n_charge > 0)
             hipLaunchKernelGGL((k_upward<double, D>), dim3((unsigned)T.n_charge), dim3(256), 0, s,
-                               T.charge_nodes.as<int32_t>(), T.offsets.as<int32_t>(),
-                               T.counts.as<int32_t>(), T.tri_xyz.as<double>(), pp->x, pp->y, pp->z,
-                               pp->vort, pp->area, T.ids.as<int32_t>(), ctx->d_vinv.as<double>(), p0,
-                               reinterpret_cast<double *>(src));
+                               T.charge_nodes.as<int32_t>(), T.nrange.as<int4>(), T.recnode.as<int32_t>(),
+                               T.tri_info.as<double>(), T.tri_xyz.as<double>(), ctx->d_vinv.as<double>(),
+                               p0, src);
         if (T.n_proxy > 0)
             hipLaunchKernelGGL((k_proxy<double, D>), dim3((unsigned)T.n_proxy), dim3(64), 0, s,
                                T.proxy_items.as<ProxyItem>(), T.ranges_b.as<Range>(), src,
-                               (const float *)nullptr, p0, T.tri_info.as<double>(),
+                               T.recnode.as<int32_t>(), (const float *)nullptr, p0, T.tri_info.as<double>(),
                                T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
         if (ev) FSS_HIP(hipEventRecord(ev[1], s));
         if (T.n_near > 0)
             hipLaunchKernelGGL((k_near<double, 2>), dim3((unsigned)T.n_near), dim3(64), 0, s,
                                T.near_items.as<NearItem>(), T.ranges_a.as<Range>(), src,
-                               T.ids.as<int32_t>(), p0, T.tri_info.as<double>(), pp->x, pp->y,
-                               pp->z, ctx->out3.as<double>());
+                               T.recid.as<int32_t>(), T.recnode.as<int32_t>(), p0, p, T.tri_info.as<double>(),
+                               pp->x, pp->y, pp->z, ost, ctx->out3.as<double>());
         if (ev) FSS_HIP(hipEventRecord(ev[2], s));
     } else {
         FSS_TRY(ctx->src32.ensure(sizeof(float4) * nsrc + sizeof(float) * nprox));
         float4 *src = ctx->src32.as<float4>();
         float *aoff = reinterpret_cast<float *>(src + nsrc);
         hipLaunchKernelGGL(k_gather32, dim3(gs), dim3(256), 0, s, pp->x, pp->y, pp->z, pp->vort,
-                           pp->area, T.ids.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, p,
-                           T.posnode.as<int32_t>(), T.tri_info.as<double>(), bltc ? 0 : 1, src,
+                           pp->area, T.recid.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, p,
+                           T.recnode.as<int32_t>(), T.tri_info.as<double>(), bltc ? 0 : 1, src,
                            aoff);
         if (bltc && T.n_charge > 0)
             hipLaunchKernelGGL((k_upward<float, D>), dim3((unsigned)T.n_charge), dim3(256), 0, s,
-                               T.charge_nodes.as<int32_t>(), T.offsets.as<int32_t>(),
-                               T.counts.as<int32_t>(), T.tri_xyz.as<double>(), pp->x, pp->y, pp->z,
-                               pp->vort, pp->area, T.ids.as<int32_t>(), ctx->d_vinv.as<double>(), p0,
-                               reinterpret_cast<float *>(src));
+                               T.charge_nodes.as<int32_t>(), T.nrange.as<int4>(), T.recnode.as<int32_t>(),
+                               T.tri_info.as<double>(), T.tri_xyz.as<double>(), ctx->d_vinv.as<double>(),
+                               p0, src);
         if (T.n_proxy > 0)
             hipLaunchKernelGGL((k_proxy<float, D>), dim3((unsigned)T.n_proxy), dim3(64), 0, s,
                                T.proxy_items.as<ProxyItem>(), T.ranges_b.as<Range>(), src,
-                               (const float *)aoff, p0, T.tri_info.as<double>(),
+                               T.recnode.as<int32_t>(), (const float *)aoff, p0, T.tri_info.as<double>(),
                                T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
         if (ev) FSS_HIP(hipEventRecord(ev[1], s));
         if (T.n_near > 0)
             hipLaunchKernelGGL((k_near<float, 2>), dim3((unsigned)T.n_near), dim3(64), 0, s,
                                T.near_items.as<NearItem>(), T.ranges_a.as<Range>(), src,
-                               T.ids.as<int32_t>(), p0, T.tri_info.as<double>(), pp->x, pp->y,
-                               pp->z, ctx->out3.as<double>());
+                               T.recid.as<int32_t>(), T.recnode.as<int32_t>(), p0, p, T.tri_info.as<double>(),
+                               pp->x, pp->y, pp->z, ost, ctx->out3.as<double>());
         if (ev) FSS_HIP(hipEventRecord(ev[2], s));
     }
     if (T.n_slot > 0)
@@ -741,7 +762,7 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
     hipLaunchKernelGGL((k_combine<D>), dim3((unsigned)div_up(T.n, 256)), dim3(256), 0, s,
                        T.inv_active.as<int32_t>(), T.nodeof.as<int32_t>(), T.node_slot.as<int32_t>(),
                        ctx->alpha.as<double>(), T.tri_xyz.as<double>(), pp->x, pp->y, pp->z,
-                       T.inst.as<Instance>(), T.ninst, T.n, p0, ctx->out3.as<double>(), bltc ? 1 : 0,
+                       T.inst.as<Instance>(), T.ninst, T.n, ost, ctx->out3.as<double>(), bltc ? 1 : 0,
                        scale, ctx->cfg.omega, out->u, out->v, out->w, out->dvort);
     if (ev) FSS_HIP(hipEventRecord(ev[3], s));
     FSS_HIP(hipGetLastError());

@@ -60,20 +60,34 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Source records (per stage): [0, p0) particle records in tree order, then
+// the p proxies of every node at p0 + node * p.
+//   Tree order: every particle has one "main" record.  Main records are
+//   sorted by the particle's path from level 0 down to the first level where
+//   it is not on a node list (then by id), so the main members of any node
+//   at any level are one contiguous range.  A particle that the reference's
+//   point location drops at some level and finds again deeper (parity
+//   assignment, icos_utils.h:139-140) is a member of deeper nodes outside
+//   that nesting: each such membership gets an extra "exception" record
+//   after the main records, grouped by node.  A node's list is its main
+//   range plus its (almost always empty) exception range.
+//   Mixed precision: a particle record holds the FP32 offset from the centre
+//   of its record node (the deepest node of its main path, or the node of an
+//   exception record); a proxy record the offset from its node's centre.
+
 // One work item of the direct-target ("near") kernel: up to 64 targets of one
 // node and the list of its class-A source ranges.
 struct NearItem {
-    int32_t tgt_start; // position in the level-ordered source array
+    int32_t tgt_start; // position of the first target record
     int32_t tgt_cnt;   // 1..64
     int32_t a_beg;     // [a_beg, a_end) into the sorted range list of class A
     int32_t a_end;
     int32_t node;      // global index of the target node
-    int32_t pad;
+    int32_t out_base;  // output slot of the first target
 };
 
-// A source range: records [start, start + count) of the level-ordered source
-// array, all belonging to (or being the proxies of) node `node`.  In mixed
-// precision the records hold FP32 offsets from that node's centre.
+// A source range: records [start, start + count), all members (or the
+// proxies) of node `node`.
 struct Range {
     int32_t start;
     int32_t count; // may carry FSS_SELF_BIT
@@ -133,7 +147,9 @@ struct Tree {
     int64_t n = 0;        // particles
     int64_t nn = 0;       // nodes on levels 0..L-1
     int64_t ntri_all = 0; // triangles on levels 0..L (geometry is built one level deeper)
-    int64_t p0 = 0;       // level-ordered particle positions (sum of list lengths)
+    int64_t p0 = 0;       // particle records: n main + exception records
+    int64_t nlist = 0;    // sum of the node list lengths (reference lists, export)
+    int64_t nslot_out = 0; // output slots of the near kernel (targets x levels with work)
     int64_t ninter = 0;
     int ngen = 0;
     std::vector<int64_t> level_base; // L+1 entries, host copy
@@ -145,9 +161,12 @@ struct Tree {
     DevBuf found;    // uint8 [L][n]
     DevBuf counts;   // int32 [nn]
     DevBuf offsets;  // int32 [nn] global start in ids
-    DevBuf ids;      // int32 [p0]
+    DevBuf ids;      // int32 [nlist]: reference node lists (ascending ids per node)
     DevBuf inter;    // int32 [7][ninter] AoS
-    DevBuf inv_active; // int32 [L][n]: position of a particle in a class-A node, or -1
+    DevBuf recid;    // int32 [p0]: particle of every record
+    DevBuf recnode;  // int32 [p0]: node whose centre is the origin of the record (mixed precision)
+    DevBuf nrange;   // int4 [nn]: {main start, main count, exception start, exception count}
+    DevBuf inv_active; // int32 [L][n]: output slot of a particle in a class-A node, or -1
     DevBuf nodeof;     // int32 [L][n]: global node of a particle per level, or -1 (dropped)
     DevBuf node_slot;  // int32 [nn]: interpolant slot of a node with class-B work, or -1
 
@@ -158,7 +177,6 @@ struct Tree {
     DevBuf proxy_slots; // ProxySlot[n_slot]
     DevBuf ranges_a;    // Range [n_ra]
     DevBuf ranges_b;    // Range [n_rb]
-    DevBuf posnode;     // int32 [p0]: node of every level-ordered position
     DevBuf inst;        // Instance[ninst]
     int ninst = 1;
     bool batched = false;

@@ -137,6 +137,106 @@ __global__ void k_level_keys(const int32_t *loc, const uint8_t *found, int64_t n
 }
 
 // ------------------------------------------------------------------
+// tree order of the source records
+// ------------------------------------------------------------------
+// The main path of a particle covers levels 0 .. e-1, e = first level where it
+// is not on a list.  Its key holds one digit per level: the level-0 node
+// (0..19, 20 = not found), then the child digit (0..3) while covered, 4 after.
+__global__ void k_main_keys(const int32_t *__restrict__ loc, const uint8_t *__restrict__ found, int64_t n,
+                            int L, unsigned long long *keys, int32_t *vals, int32_t *ncnt, int32_t *ecnt,
+                            unsigned long long *nexc)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    unsigned long long key = 0;
+    bool cov = true;
+    int prev = 0;
+    for (int l = 0; l < L; l++) {
+        bool f = found[(int64_t)l * n + id] != 0;
+        int lc = loc[(int64_t)l * n + id];
+        cov = cov && f;
+        int digit;
+        if (l == 0) digit = cov ? lc : 20;
+        else digit = cov ? lc - 4 * prev : 4;
+        key = key * (l == 0 ? 1ull : 5ull) + (unsigned long long)digit;
+        int64_t g = level_offset(l) + lc;
+        if (cov) atomicAdd(&ncnt[g], 1);
+        else if (f) {
+            atomicAdd(&ecnt[g], 1);
+            atomicAdd(nexc, 1ull);
+        }
+        prev = lc;
+    }
+    keys[id] = key;
+    vals[id] = (int32_t)id;
+}
+
+// start of every node's main range; record node (deepest node of the main
+// path) and particle of every main record
+__global__ void k_main_ranges(const int32_t *__restrict__ perm, const int32_t *__restrict__ loc,
+                              const uint8_t *__restrict__ found, int64_t n, int L, int4 *nrange,
+                              int32_t *recid, int32_t *recnode)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t id = perm[i];
+    int64_t pv = i > 0 ? perm[i - 1] : -1;
+    bool cov = true, pcov = pv >= 0;
+    int32_t deepest = 0;
+    for (int l = 0; l < L; l++) {
+        cov = cov && found[(int64_t)l * n + id];
+        if (!cov) break;
+        int lc = loc[(int64_t)l * n + id];
+        pcov = pcov && found[(int64_t)l * n + pv];
+        int64_t g = level_offset(l) + lc;
+        if (!pcov || loc[(int64_t)l * n + pv] != lc) nrange[g].x = (int32_t)i;
+        deepest = (int32_t)g;
+    }
+    recid[i] = (int32_t)id;
+    recnode[i] = deepest;
+}
+
+// exception memberships: (node, particle) of every level a particle is on a
+// list outside its main path
+__global__ void k_exc_fill(const int32_t *__restrict__ loc, const uint8_t *__restrict__ found, int64_t n,
+                           int L, unsigned long long *counter, unsigned long long *keys, int32_t *vals)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    bool cov = true;
+    for (int l = 0; l < L; l++) {
+        bool f = found[(int64_t)l * n + id] != 0;
+        cov = cov && f;
+        if (!cov && f) {
+            unsigned long long q = atomicAdd(counter, 1ull);
+            int64_t g = level_offset(l) + loc[(int64_t)l * n + id];
+            keys[q] = ((unsigned long long)g << 32) | (unsigned long long)id;
+            vals[q] = (int32_t)id;
+        }
+    }
+}
+
+__global__ void k_exc_ranges(const unsigned long long *__restrict__ keys, const int32_t *__restrict__ vals,
+                             int64_t ne, int64_t base, int4 *nrange, int32_t *recid, int32_t *recnode)
+{
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ne) return;
+    int64_t g = (int64_t)(keys[q] >> 32);
+    if (q == 0 || (int64_t)(keys[q - 1] >> 32) != g) nrange[g].z = (int32_t)(base + q);
+    recid[base + q] = vals[q];
+    recnode[base + q] = (int32_t)g;
+}
+
+__global__ void k_nrange_counts(const int32_t *__restrict__ ncnt, const int32_t *__restrict__ ecnt,
+                                int64_t nn, int4 *nrange)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nn) return;
+    nrange[g].y = ncnt[g];
+    nrange[g].w = ecnt[g];
+}
+
+// ------------------------------------------------------------------
 // traversal (fast_rhs.h:16-91)
 // ------------------------------------------------------------------
 enum : int8_t { ACT_SKIP = -1, ACT_SPLIT_T = 4, ACT_SPLIT_S = 5 };
@@ -244,9 +344,10 @@ __global__ void k_plan_keys(const int32_t *__restrict__ inter, int64_t m, uint32
     if (cls == 0) atomicAdd(nA, 1ull);
 }
 
-__global__ void k_plan_ranges(const int32_t *__restrict__ inter, const int32_t *__restrict__ sorted_idx,
-                              int64_t m, const int32_t *__restrict__ offsets, int bltc, int p,
-                              int64_t p0, Range *ranges, unsigned long long *pair_acc)
+// ranges per sorted interaction: 1, or 2 for particle sources whose node has
+// exception members (main range + exception range)
+__global__ void k_plan_rcount(const int32_t *__restrict__ inter, const int32_t *__restrict__ sorted_idx,
+                              int64_t m, const int4 *__restrict__ nrange, int bltc, int32_t *nr)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
@@ -254,35 +355,51 @@ __global__ void k_plan_ranges(const int32_t *__restrict__ inter, const int32_t *
     int type = o[6];
     int64_t gs = level_offset(o[1]) + o[3];
     bool proxy_src = bltc && (type == 1 || type == 3);
+    nr[k] = (!proxy_src && nrange[gs].w > 0) ? 2 : 1;
+}
+
+__global__ void k_plan_ranges(const int32_t *__restrict__ inter, const int32_t *__restrict__ sorted_idx,
+                              const uint32_t *__restrict__ sorted_keys, const int32_t *__restrict__ rpos,
+                              int64_t m, const int4 *__restrict__ nrange, int bltc, int p, int64_t p0,
+                              Range *ranges, uint32_t *rkeys, unsigned long long *pair_acc)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int32_t *o = inter + 7 * sorted_idx[k];
+    int type = o[6];
+    int64_t gs = level_offset(o[1]) + o[3];
+    bool proxy_src = bltc && (type == 1 || type == 3);
+    bool clsA = (type == 0 || type == 1);
+    // pair accounting: near = direct-target evaluations, far = proxy-target
+    unsigned long long tgt = clsA ? (unsigned long long)o[4] : (unsigned long long)p;
+    unsigned long long srcn = proxy_src ? (unsigned long long)p : (unsigned long long)o[5];
+    atomicAdd(&pair_acc[clsA ? 0 : 1], tgt * srcn);
+    // particle sources of a node that contains the target node: the self pair
+    // may lie in these ranges
+    int lt = o[0], ls = o[1];
+    int self = (clsA && !proxy_src && ls <= lt && (o[2] >> (2 * (lt - ls))) == o[3]) ? FSS_SELF_BIT : 0;
+    int32_t q = rpos[k];
     Range r;
     r.node = (int32_t)gs;
     r.pad = 0;
     if (proxy_src) {
         r.start = (int32_t)(p0 + gs * p);
         r.count = p;
-    } else {
-        r.start = offsets[gs];
-        r.count = o[5];
+        ranges[q] = r;
+        rkeys[q] = sorted_keys[k];
+        return;
     }
-    // pair accounting: near = direct-target evaluations, far = proxy-target
-    bool clsA = (type == 0 || type == 1);
-    unsigned long long tgt = clsA ? (unsigned long long)o[4] : (unsigned long long)p;
-    atomicAdd(&pair_acc[clsA ? 0 : 1], tgt * (unsigned long long)r.count);
-    // particle sources of a node that contains the target node: the self pair
-    // lies in this range
-    int lt = o[0], ls = o[1];
-    if (clsA && !proxy_src && ls <= lt && (o[2] >> (2 * (lt - ls))) == o[3]) r.count |= FSS_SELF_BIT;
-    ranges[k] = r;
-}
-
-// node of every level-ordered position (one block per node)
-__global__ void k_plan_posnode(const int32_t *__restrict__ counts, const int32_t *__restrict__ offsets,
-                               uint32_t nn, int32_t *posnode)
-{
-    int64_t g = blockIdx.x;
-    if (g >= nn) return;
-    int32_t c = counts[g], off = offsets[g];
-    for (int32_t t = threadIdx.x; t < c; t += blockDim.x) posnode[off + t] = (int32_t)g;
+    int4 nr = nrange[gs];
+    r.start = nr.x;
+    r.count = nr.y | self;
+    ranges[q] = r;
+    rkeys[q] = sorted_keys[k];
+    if (nr.w > 0) {
+        r.start = nr.z;
+        r.count = nr.w | self;
+        ranges[q + 1] = r;
+        rkeys[q + 1] = sorted_keys[k];
+    }
 }
 
 __global__ void k_plan_segments(const uint32_t *__restrict__ keys, int64_t m, uint32_t nn,
@@ -295,22 +412,24 @@ __global__ void k_plan_segments(const uint32_t *__restrict__ keys, int64_t m, ui
     if (k == m - 1 || keys[k + 1] != key) end[key] = (int32_t)(k + 1);
 }
 
-// per node: number of near chunks and whether it has proxy work
+// per node: number of near chunks (main and exception targets), output
+// slots of its targets, and whether it has proxy work
 __global__ void k_plan_nodes(const int32_t *__restrict__ beg, const int32_t *__restrict__ end,
-                             const int32_t *__restrict__ counts, uint32_t nn, int32_t *nchunks,
-                             int32_t *hasB)
+                             const int4 *__restrict__ nrange, uint32_t nn, int32_t *nchunks,
+                             int32_t *nout, int32_t *hasB)
 {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nn) return;
     bool a = end[g] > beg[g];
     bool b = end[nn + g] > beg[nn + g];
-    int32_t c = counts[g];
-    nchunks[g] = a ? (int32_t)((c + 63) / 64) : 0;
+    int4 r = nrange[g];
+    nchunks[g] = a ? (int32_t)((r.y + 63) / 64 + (r.w + 63) / 64) : 0;
+    nout[g] = a ? r.y + r.w : 0;
     hasB[g] = b ? 1 : 0;
 }
 
 __global__ void k_plan_items(const int32_t *__restrict__ beg, const int32_t *__restrict__ end,
-                             const int32_t *__restrict__ counts, const int32_t *__restrict__ offsets,
+                             const int4 *__restrict__ nrange, const int32_t *__restrict__ out_pos,
                              const int32_t *__restrict__ chunk_pos, const int32_t *__restrict__ bslot,
                              const int32_t *__restrict__ hasB, uint32_t nn, int64_t nA,
                              NearItem *near, ProxyItem *prox, int32_t *node_slot)
@@ -329,17 +448,21 @@ __global__ void k_plan_items(const int32_t *__restrict__ beg, const int32_t *__r
         prox[bslot[g]] = it;
     }
     if (!a) return;
-    int32_t c = counts[g], off = offsets[g];
-    int32_t nch = (c + 63) / 64;
-    for (int32_t q = 0; q < nch; q++) {
-        NearItem it;
-        it.tgt_start = off + 64 * q;
-        it.tgt_cnt = min(64, c - 64 * q);
-        it.a_beg = beg[g];
-        it.a_end = end[g];
-        it.node = (int32_t)g;
-        it.pad = 0;
-        near[chunk_pos[g] + q] = it;
+    int4 r = nrange[g];
+    int32_t item = chunk_pos[g], ob = out_pos[g];
+    for (int part = 0; part < 2; part++) {
+        int32_t c = part ? r.w : r.y, off = part ? r.z : r.x;
+        for (int32_t q = 0; 64 * q < c; q++) {
+            NearItem it;
+            it.tgt_start = off + 64 * q;
+            it.tgt_cnt = min(64, c - 64 * q);
+            it.a_beg = beg[g];
+            it.a_end = end[g];
+            it.node = (int32_t)g;
+            it.out_base = ob + 64 * q;
+            near[item++] = it;
+        }
+        ob += c;
     }
 }
 
@@ -353,20 +476,17 @@ __global__ void k_plan_nodeof(const int32_t *__restrict__ loc, const uint8_t *__
     nodeof[i] = found[i] ? (int32_t)(level_offset(l) + loc[i]) : -1;
 }
 
-// inverse map for the final per-particle reduction: for every node with any
-// work, the level-ordered position of each of its particles.
-__global__ void k_plan_inverse(const int32_t *__restrict__ nchunks, const int32_t *__restrict__ counts,
-                               const int32_t *__restrict__ offsets, const int32_t *__restrict__ ids,
-                               uint32_t nn, int L, int64_t n, int32_t *inv_active)
+// inverse map for the final per-particle reduction: the output slot of every
+// near target, per level (one 64-thread block per near item)
+__global__ void k_plan_inverse(const NearItem *__restrict__ items, const int32_t *__restrict__ recid, int L,
+                               int64_t n, int32_t *inv_active)
 {
-    // one block per node
-    int64_t g = blockIdx.x;
-    if (g >= nn || nchunks[g] == 0) return;
+    const NearItem it = items[blockIdx.x];
+    int t = threadIdx.x;
+    if (t >= it.tgt_cnt) return;
     int lev = 0;
-    while (lev + 1 < L && level_offset(lev + 1) <= g) lev++;
-    int32_t c = counts[g], off = offsets[g];
-    for (int32_t t = threadIdx.x; t < c; t += blockDim.x)
-        inv_active[(int64_t)lev * n + ids[off + t]] = off + t;
+    while (lev + 1 < L && level_offset(lev + 1) <= it.node) lev++;
+    inv_active[(int64_t)lev * n + recid[it.tgt_start + t]] = it.out_base + t;
 }
 
 // nodes whose proxies carry modified charges (BLTC: sources of pc and cc)
@@ -435,6 +555,23 @@ static fss_status cub_sort_pairs(DevBuf &tmp, const uint32_t *kin, uint32_t *kou
     return FSS_OK;
 }
 
+static fss_status cub_sort_pairs64(DevBuf &tmp, const unsigned long long *kin, unsigned long long *kout,
+                                   const int32_t *vin, int32_t *vout, int64_t n, int bits, hipStream_t s)
+{
+    size_t bytes = 0;
+    FSS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0, bits, s));
+    FSS_TRY(tmp.ensure(bytes));
+    FSS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, kin, kout, vin, vout, (int)n, 0, bits, s));
+    return FSS_OK;
+}
+
+static int bits_for64(uint64_t maxkey)
+{
+    int b = 1;
+    while (b < 64 && ((uint64_t)1 << b) <= maxkey) b++;
+    return b;
+}
+
 static fss_status cub_exclusive_sum(DevBuf &tmp, const int32_t *in, int32_t *out, int64_t n,
                                     hipStream_t s)
 {
@@ -455,7 +592,8 @@ static fss_status d2h(void *dst, const void *src, size_t bytes, hipStream_t s)
 void tree_release(Tree &t)
 {
     DevBuf *bufs[] = {&t.tri_xyz, &t.tri_info, &t.loc, &t.found, &t.counts, &t.offsets, &t.ids,
-                      &t.inter, &t.inv_active, &t.nodeof, &t.node_slot, &t.proxy_slots, &t.posnode,
+                      &t.inter, &t.inv_active, &t.nodeof, &t.node_slot, &t.proxy_slots, &t.recid,
+                      &t.recnode, &t.nrange,
                       &t.near_items, &t.proxy_items, &t.ranges_a,
                       &t.ranges_b, &t.charge_nodes, &t.proxy_xyz};
     for (DevBuf *b : bufs) b->release();
@@ -540,14 +678,69 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
         for (int l = 0; l < L; l++) T.level_base[l] = o[l];
         T.level_base[L] = (int64_t)lasto[0] + lastc[0];
     }
-    T.p0 = T.level_base[L];
-    FSS_TRY(T.ids.ensure(sizeof(int32_t) * std::max<int64_t>(1, T.p0)));
+    T.nlist = T.level_base[L];
+    FSS_TRY(T.ids.ensure(sizeof(int32_t) * std::max<int64_t>(1, T.nlist)));
     for (int l = 0; l < L; l++) {
         int64_t cnt = T.level_base[l + 1] - T.level_base[l];
         T.stats.dropped[l] = n - cnt;
         if (cnt > 0)
             FSS_HIP(hipMemcpyAsync(T.ids.as<int32_t>() + T.level_base[l], S.d.as<int32_t>() + (int64_t)l * n,
                                    sizeof(int32_t) * cnt, hipMemcpyDeviceToDevice, s));
+    }
+
+    // ---- tree order of the source records ----
+    {
+        DevBuf k0, k1, v0, v1, cnt2, nexc;
+        FSS_TRY(k0.ensure(sizeof(unsigned long long) * n));
+        FSS_TRY(k1.ensure(sizeof(unsigned long long) * n));
+        FSS_TRY(v0.ensure(sizeof(int32_t) * n));
+        FSS_TRY(v1.ensure(sizeof(int32_t) * n));
+        FSS_TRY(cnt2.ensure(sizeof(int32_t) * 2 * T.nn));
+        FSS_TRY(nexc.ensure(sizeof(unsigned long long)));
+        FSS_HIP(hipMemsetAsync(cnt2.p, 0, sizeof(int32_t) * 2 * T.nn, s));
+        FSS_HIP(hipMemsetAsync(nexc.p, 0, sizeof(unsigned long long), s));
+        int32_t *ncnt = cnt2.as<int32_t>(), *ecnt = ncnt + T.nn;
+        unsigned gp = (unsigned)div_up(n, 256);
+        hipLaunchKernelGGL(k_main_keys, dim3(gp), dim3(256), 0, s, T.loc.as<int32_t>(), T.found.as<uint8_t>(),
+                           n, L, k0.as<unsigned long long>(), v0.as<int32_t>(), ncnt, ecnt,
+                           nexc.as<unsigned long long>());
+        uint64_t maxkey = 21;
+        for (int l = 1; l < L; l++) maxkey *= 5;
+        FSS_TRY(cub_sort_pairs64(S.cub, k0.as<unsigned long long>(), k1.as<unsigned long long>(),
+                                 v0.as<int32_t>(), v1.as<int32_t>(), n, bits_for64(maxkey), s));
+        unsigned long long ne = 0;
+        FSS_TRY(d2h(&ne, nexc.p, sizeof(ne), s));
+        T.p0 = n + (int64_t)ne;
+        if (T.p0 + T.nn * p > INT32_MAX) return set_error("tree: source records exceed 32-bit positions"), FSS_ERR_INVALID;
+        FSS_TRY(T.recid.ensure(sizeof(int32_t) * T.p0));
+        FSS_TRY(T.recnode.ensure(sizeof(int32_t) * T.p0));
+        FSS_TRY(T.nrange.ensure(sizeof(int4) * T.nn));
+        FSS_HIP(hipMemsetAsync(T.nrange.p, 0, sizeof(int4) * T.nn, s));
+        hipLaunchKernelGGL(k_nrange_counts, dim3((unsigned)div_up(T.nn, 256)), dim3(256), 0, s, ncnt, ecnt, T.nn,
+                           T.nrange.as<int4>());
+        hipLaunchKernelGGL(k_main_ranges, dim3(gp), dim3(256), 0, s, v1.as<int32_t>(), T.loc.as<int32_t>(),
+                           T.found.as<uint8_t>(), n, L, T.nrange.as<int4>(), T.recid.as<int32_t>(),
+                           T.recnode.as<int32_t>());
+        if (ne > 0) {
+            // exception memberships, sorted by (node, particle)
+            DevBuf ek0, ek1, ev0, ev1;
+            FSS_TRY(ek0.ensure(sizeof(unsigned long long) * ne));
+            FSS_TRY(ek1.ensure(sizeof(unsigned long long) * ne));
+            FSS_TRY(ev0.ensure(sizeof(int32_t) * ne));
+            FSS_TRY(ev1.ensure(sizeof(int32_t) * ne));
+            FSS_HIP(hipMemsetAsync(nexc.p, 0, sizeof(unsigned long long), s));
+            hipLaunchKernelGGL(k_exc_fill, dim3(gp), dim3(256), 0, s, T.loc.as<int32_t>(), T.found.as<uint8_t>(),
+                               n, L, nexc.as<unsigned long long>(), ek0.as<unsigned long long>(),
+                               ev0.as<int32_t>());
+            FSS_TRY(cub_sort_pairs64(S.cub, ek0.as<unsigned long long>(), ek1.as<unsigned long long>(),
+                                     ev0.as<int32_t>(), ev1.as<int32_t>(), (int64_t)ne,
+                                     32 + bits_for64((uint64_t)T.nn), s));
+            hipLaunchKernelGGL(k_exc_ranges, dim3((unsigned)div_up((int64_t)ne, 256)), dim3(256), 0, s,
+                               ek1.as<unsigned long long>(), ev1.as<int32_t>(), (int64_t)ne, n,
+                               T.nrange.as<int4>(), T.recid.as<int32_t>(), T.recnode.as<int32_t>());
+        }
+        FSS_HIP(hipGetLastError());
+        FSS_HIP(hipStreamSynchronize(s));
     }
 
     // ---- traversal ----
@@ -630,13 +823,13 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
                        T.tri_xyz.as<double>(), ctx->d_pts.as<double>(), p, T.nn, bltc ? 1 : 0,
                        T.proxy_xyz.as<double4>());
     int64_t m = ninter;
-    DevBuf keys, keys2, vals, vals2, acc, begend, nch, chpos, hasB, bslot;
+    DevBuf keys, keys2, vals, vals2, acc, begend, nch, chpos, hasB, bslot, nout, opos, rcnt, rpos, rkeys;
     FSS_TRY(acc.ensure(sizeof(unsigned long long) * 3));
     FSS_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long) * 3, s));
     FSS_TRY(begend.ensure(sizeof(int32_t) * 4 * T.nn));
     FSS_HIP(hipMemsetAsync(begend.p, 0, sizeof(int32_t) * 4 * T.nn, s));
     int32_t *beg = begend.as<int32_t>(), *end = beg + 2 * T.nn;
-    int64_t nA = 0;
+    int64_t nA = 0, nR = 0;
     if (m > 0) {
         FSS_TRY(keys.ensure(sizeof(uint32_t) * m));
         FSS_TRY(keys2.ensure(sizeof(uint32_t) * m));
@@ -649,16 +842,36 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
                                vals2.as<int32_t>(), m, bits_for(2ull * nn), s));
         unsigned long long hA;
         FSS_TRY(d2h(&hA, acc.as<unsigned long long>() + 2, 8, s));
-        nA = (int64_t)hA;
-        FSS_TRY(T.ranges_a.ensure(sizeof(Range) * m));
+        int64_t mA = (int64_t)hA; // class-A interactions (sorted first)
+        // one range per interaction, two where a particle source node has
+        // exception members
+        FSS_TRY(rcnt.ensure(sizeof(int32_t) * m));
+        FSS_TRY(rpos.ensure(sizeof(int32_t) * (m + 1)));
+        hipLaunchKernelGGL(k_plan_rcount, dim3(gb), dim3(256), 0, s, T.inter.as<int32_t>(), vals2.as<int32_t>(),
+                           m, T.nrange.as<int4>(), bltc ? 1 : 0, rcnt.as<int32_t>());
+        FSS_TRY(cub_exclusive_sum(S.cub, rcnt.as<int32_t>(), rpos.as<int32_t>(), m, s));
+        int32_t rt[2];
+        FSS_TRY(d2h(&rt[0], rpos.as<int32_t>() + m - 1, 4, s));
+        FSS_TRY(d2h(&rt[1], rcnt.as<int32_t>() + m - 1, 4, s));
+        nR = (int64_t)rt[0] + rt[1];
+        if (mA < m) {
+            int32_t ra;
+            FSS_TRY(d2h(&ra, rpos.as<int32_t>() + mA, 4, s));
+            nA = ra;
+        } else {
+            nA = nR;
+        }
+        FSS_TRY(T.ranges_a.ensure(sizeof(Range) * nR));
+        FSS_TRY(rkeys.ensure(sizeof(uint32_t) * nR));
         hipLaunchKernelGGL(k_plan_ranges, dim3(gb), dim3(256), 0, s, T.inter.as<int32_t>(),
-                           vals2.as<int32_t>(), m, T.offsets.as<int32_t>(), bltc ? 1 : 0, p, T.p0,
-                           T.ranges_a.as<Range>(), acc.as<unsigned long long>());
-        hipLaunchKernelGGL(k_plan_segments, dim3(gb), dim3(256), 0, s, keys2.as<uint32_t>(), m, nn,
-                           beg, end);
+                           vals2.as<int32_t>(), keys2.as<uint32_t>(), rpos.as<int32_t>(), m,
+                           T.nrange.as<int4>(), bltc ? 1 : 0, p, T.p0, T.ranges_a.as<Range>(),
+                           rkeys.as<uint32_t>(), acc.as<unsigned long long>());
+        hipLaunchKernelGGL(k_plan_segments, dim3((unsigned)div_up(nR, 256)), dim3(256), 0, s,
+                           rkeys.as<uint32_t>(), nR, nn, beg, end);
     }
     T.n_ra = nA;
-    T.n_rb = m - nA;
+    T.n_rb = nR - nA;
     {
         unsigned long long pa[2];
         FSS_TRY(d2h(pa, acc.p, 16, s));
@@ -671,19 +884,25 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
     FSS_TRY(chpos.ensure(sizeof(int32_t) * T.nn));
     FSS_TRY(hasB.ensure(sizeof(int32_t) * T.nn));
     FSS_TRY(bslot.ensure(sizeof(int32_t) * T.nn));
+    FSS_TRY(nout.ensure(sizeof(int32_t) * T.nn));
+    FSS_TRY(opos.ensure(sizeof(int32_t) * T.nn));
     unsigned gn = (unsigned)div_up(T.nn, 256);
-    hipLaunchKernelGGL(k_plan_nodes, dim3(gn), dim3(256), 0, s, beg, end, T.counts.as<int32_t>(), nn,
-                       nch.as<int32_t>(), hasB.as<int32_t>());
+    hipLaunchKernelGGL(k_plan_nodes, dim3(gn), dim3(256), 0, s, beg, end, T.nrange.as<int4>(), nn,
+                       nch.as<int32_t>(), nout.as<int32_t>(), hasB.as<int32_t>());
     FSS_TRY(cub_exclusive_sum(S.cub, nch.as<int32_t>(), chpos.as<int32_t>(), T.nn, s));
     FSS_TRY(cub_exclusive_sum(S.cub, hasB.as<int32_t>(), bslot.as<int32_t>(), T.nn, s));
+    FSS_TRY(cub_exclusive_sum(S.cub, nout.as<int32_t>(), opos.as<int32_t>(), T.nn, s));
     {
-        int32_t t4[4];
-        FSS_TRY(d2h(&t4[0], chpos.as<int32_t>() + T.nn - 1, 4, s));
-        FSS_TRY(d2h(&t4[1], nch.as<int32_t>() + T.nn - 1, 4, s));
-        FSS_TRY(d2h(&t4[2], bslot.as<int32_t>() + T.nn - 1, 4, s));
-        FSS_TRY(d2h(&t4[3], hasB.as<int32_t>() + T.nn - 1, 4, s));
-        T.n_near = (int64_t)t4[0] + t4[1];
-        T.n_slot = (int64_t)t4[2] + t4[3];
+        int32_t t6[6];
+        FSS_TRY(d2h(&t6[0], chpos.as<int32_t>() + T.nn - 1, 4, s));
+        FSS_TRY(d2h(&t6[1], nch.as<int32_t>() + T.nn - 1, 4, s));
+        FSS_TRY(d2h(&t6[2], bslot.as<int32_t>() + T.nn - 1, 4, s));
+        FSS_TRY(d2h(&t6[3], hasB.as<int32_t>() + T.nn - 1, 4, s));
+        FSS_TRY(d2h(&t6[4], opos.as<int32_t>() + T.nn - 1, 4, s));
+        FSS_TRY(d2h(&t6[5], nout.as<int32_t>() + T.nn - 1, 4, s));
+        T.n_near = (int64_t)t6[0] + t6[1];
+        T.n_slot = (int64_t)t6[2] + t6[3];
+        T.nslot_out = (int64_t)t6[4] + t6[5];
     }
     FSS_TRY(T.near_items.ensure(sizeof(NearItem) * std::max<int64_t>(1, T.n_near)));
     DevBuf prox_one;
@@ -694,8 +913,8 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
     if (T.n_rb > 0)
         FSS_HIP(hipMemcpyAsync(T.ranges_b.p, T.ranges_a.as<Range>() + nA, sizeof(Range) * T.n_rb,
                                hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_plan_items, dim3(gn), dim3(256), 0, s, beg, end, T.counts.as<int32_t>(),
-                       T.offsets.as<int32_t>(), chpos.as<int32_t>(), bslot.as<int32_t>(),
+    hipLaunchKernelGGL(k_plan_items, dim3(gn), dim3(256), 0, s, beg, end, T.nrange.as<int4>(),
+                       opos.as<int32_t>(), chpos.as<int32_t>(), bslot.as<int32_t>(),
                        hasB.as<int32_t>(), nn, nA, T.near_items.as<NearItem>(),
                        prox_one.as<ProxyItem>(), T.node_slot.as<int32_t>());
     // split long class-B lists into chunks of about FSS_PROXY_CHUNK sources
@@ -738,14 +957,11 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
     FSS_TRY(T.nodeof.ensure(sizeof(int32_t) * (size_t)L * n));
     hipLaunchKernelGGL(k_plan_nodeof, dim3((unsigned)div_up((int64_t)L * n, 256)), dim3(256), 0, s,
                        T.loc.as<int32_t>(), T.found.as<uint8_t>(), L, n, T.nodeof.as<int32_t>());
-    FSS_TRY(T.posnode.ensure(sizeof(int32_t) * std::max<int64_t>(1, T.p0)));
-    hipLaunchKernelGGL(k_plan_posnode, dim3(nn), dim3(256), 0, s, T.counts.as<int32_t>(),
-                       T.offsets.as<int32_t>(), nn, T.posnode.as<int32_t>());
     FSS_TRY(T.inv_active.ensure(sizeof(int32_t) * (size_t)L * n));
     FSS_HIP(hipMemsetAsync(T.inv_active.p, 0xff, sizeof(int32_t) * (size_t)L * n, s));
-    hipLaunchKernelGGL(k_plan_inverse, dim3(nn), dim3(256), 0, s, nch.as<int32_t>(),
-                       T.counts.as<int32_t>(), T.offsets.as<int32_t>(), T.ids.as<int32_t>(), nn, L, n,
-                       T.inv_active.as<int32_t>());
+    if (T.n_near > 0)
+        hipLaunchKernelGGL(k_plan_inverse, dim3((unsigned)T.n_near), dim3(64), 0, s,
+                           T.near_items.as<NearItem>(), T.recid.as<int32_t>(), L, n, T.inv_active.as<int32_t>());
     // charge nodes (BLTC)
     T.n_charge = 0;
     if (bltc && m > 0) {
@@ -797,7 +1013,7 @@ fss_status tree_export(fss_ctx *ctx, int32_t *loc, uint8_t *found, int32_t *coun
     if (found) FSS_TRY(d2h(found, T.found.p, (size_t)T.L * T.n, s));
     if (counts) FSS_TRY(d2h(counts, T.counts.p, sizeof(int32_t) * T.nn, s));
     if (offsets) FSS_TRY(d2h(offsets, T.offsets.p, sizeof(int32_t) * T.nn, s));
-    if (ids && T.p0 > 0) FSS_TRY(d2h(ids, T.ids.p, sizeof(int32_t) * T.p0, s));
+    if (ids && T.nlist > 0) FSS_TRY(d2h(ids, T.ids.p, sizeof(int32_t) * T.nlist, s));
     if (inter && T.ninter > 0) FSS_TRY(d2h(inter, T.inter.p, sizeof(int32_t) * 7 * T.ninter, s));
     if (tri_info) FSS_TRY(d2h(tri_info, T.tri_info.p, sizeof(double) * 4 * T.ntri_all, s));
     if (tri_xyz) FSS_TRY(d2h(tri_xyz, T.tri_xyz.p, sizeof(double) * 9 * T.ntri_all, s));
