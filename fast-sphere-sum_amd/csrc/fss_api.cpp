@@ -350,6 +350,14 @@ fss_status fss_autotune_levels(fss_ctx *ctx, const fss_particles *p, int lmin, i
 
 int fss_levels_for(int64_t n, double leaf_occupancy) { return levels_for(n, leaf_occupancy); }
 
+fss_status fss_set_near_tile(fss_ctx *ctx, int targets_per_lane)
+{
+    if (!ctx || (targets_per_lane != 0 && targets_per_lane != 2 && targets_per_lane != 4))
+        return set_error("near tile: 2 or 4 targets per lane (0 = default)"), FSS_ERR_INVALID;
+    ctx->near_r = targets_per_lane;
+    return FSS_OK;
+}
+
 fss_status fss_batch_build(fss_ctx *ctx, int ninst, const int64_t *n, const int *tree_levels,
                            const double *x, const double *y, const double *z)
 {
@@ -372,7 +380,13 @@ fss_status fss_tree_stats_get(fss_ctx *ctx, fss_tree_stats *out)
 {
     if (!ctx || !out) return set_error("null argument"), FSS_ERR_INVALID;
     if (!ctx->tree.built) return set_error("no tree"), FSS_ERR_STATE;
-    *out = ctx->tree.stats;
+    const Tree &T = ctx->tree;
+    *out = T.stats;
+    out->n_records = T.p0;
+    out->n_near_items = T.n_near;
+    out->n_proxy_items = T.n_proxy;
+    out->n_interpolants = T.n_slot;
+    out->n_charge_nodes = T.n_charge;
     return FSS_OK;
 }
 

@@ -45,7 +45,8 @@ template <class T> fss_status up(DevBuf &d, const std::vector<T> &h, hipStream_t
 struct Part {
     int L;
     int64_t n, nn, p0, n_charge, nslot_out;
-    std::vector<int32_t> recid, recnode, counts, inv, nodeof, node_slot, charge;
+    std::vector<int32_t> recid, recnode, counts, inv, nodeof, node_slot, charge, oslot;
+    std::vector<uint8_t> rinfo;
     std::vector<int4> nrange;
     std::vector<double> tri_info, tri_xyz;
     std::vector<double4> proxy;
@@ -83,6 +84,8 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         P.stats = T.stats;
         FSS_TRY(down(P.recid, T.recid, T.p0, s));
         FSS_TRY(down(P.recnode, T.recnode, T.p0, s));
+        FSS_TRY(down(P.rinfo, T.rinfo, T.p0, s));
+        FSS_TRY(down(P.oslot, T.oslot, T.nn, s));
         FSS_TRY(down(P.nrange, T.nrange, T.nn, s));
         FSS_TRY(down(P.counts, T.counts, T.nn, s));
         FSS_TRY(down(P.inv, T.inv_active, (int64_t)T.L * T.n, s));
@@ -139,7 +142,8 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         r.node = (int32_t)(r.node + nbase[k]);
         return r;
     };
-    std::vector<int32_t> recid, recnode, counts, inv, nodeof, node_slot, charge;
+    std::vector<int32_t> recid, recnode, counts, inv, nodeof, node_slot, charge, oslot;
+    std::vector<uint8_t> rinfo;
     std::vector<int4> nrange;
     std::vector<double> tri_info, tri_xyz;
     std::vector<double4> proxy;
@@ -153,6 +157,8 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         Part &P = parts[k];
         for (int32_t v : P.recid) recid.push_back((int32_t)(v + pbase[k]));
         for (int32_t v : P.recnode) recnode.push_back((int32_t)(v + nbase[k]));
+        rinfo.insert(rinfo.end(), P.rinfo.begin(), P.rinfo.end());
+        for (int32_t v : P.oslot) oslot.push_back(v >= 0 ? (int32_t)(v + obase[k]) : -1);
         for (int4 r : P.nrange) {
             r.x = (int32_t)(r.x + sbase[k]);
             r.z = (int32_t)(r.z + sbase[k]);
@@ -189,7 +195,7 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         }
         for (const Range &r : P.ra) ra.push_back(remap_range(r, k));
         for (const Range &r : P.rb) rb.push_back(remap_range(r, k));
-        inst[k] = Instance{pbase[k], P.n, lbase[k], P.L, 0};
+        inst[k] = Instance{pbase[k], P.n, lbase[k], P.L, 0, sbase[k], nbase[k]};
         tot.n_interactions += P.stats.n_interactions;
         for (int t = 0; t < 4; t++) tot.n_type[t] += P.stats.n_type[t];
         tot.pairs_near += P.stats.pairs_near;
@@ -223,6 +229,8 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
     T.stats = tot;
     FSS_TRY(up(T.recid, recid, s));
     FSS_TRY(up(T.recnode, recnode, s));
+    FSS_TRY(up(T.rinfo, rinfo, s));
+    FSS_TRY(up(T.oslot, oslot, s));
     FSS_TRY(up(T.nrange, nrange, s));
     FSS_TRY(up(T.counts, counts, s));
     FSS_TRY(up(T.inv_active, inv, s));
@@ -244,6 +252,7 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
     T.ids.release();
     T.offsets.release();
     FSS_HIP(hipStreamSynchronize(s));
+    FSS_TRY(plan_finish(ctx));
     T.built = true;
     return FSS_OK;
 }

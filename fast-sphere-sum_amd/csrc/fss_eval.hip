@@ -27,6 +27,7 @@
 // that skips the self pair by particle id, as the reference does.
 #include <cmath>
 
+#include "fss_geom.h"
 #include "fss_internal.h"
 
 namespace fss {
@@ -38,13 +39,11 @@ constexpr int NCH = 256;
 // ------------------------------------------------------------------
 __device__ inline double rcp64(double d)
 {
-    // v_rcp_f64 seed + two Newton-Raphson steps (error ~1 ulp)
+    // v_rcp_f64 seed (max relative error 4.6e-8 measured on gfx950) and one
+    // Newton-Raphson step: max relative error 2.2e-15 (about 10 ulp)
     double r = __builtin_amdgcn_rcp(d);
     double e = fma(-d, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-d, r, 1.0);
-    r = fma(r, e, r);
-    return r;
+    return fma(r, e, r);
 }
 
 __device__ inline float rcp32(float d) { return __builtin_amdgcn_rcpf(d); }
@@ -83,13 +82,14 @@ __device__ inline void pair_eval(const double4 &xt, const double4 &s, double, do
     S2 = fma(c, s.z - xt.z, S2);
 }
 
-// FP32: records are offsets from a common node centre; 1 - x.y = |d|^2/2 +
-// off, where off = (1 - |y|^2)/2 for a target off the sphere (planar proxy).
-__device__ inline void pair_eval(const float4 &xt, const float4 &s, float off, float &S0, float &S1,
+// FP32: records are offsets from a common node centre and carry the doubled
+// weight 2 w; 2 (1 - x.y) = |d|^2 + off2, where off2 = 1 - |y|^2 for a target
+// off the sphere (planar proxy), so c = 2 w / (|d|^2 + off2) = w / (1 - x.y).
+__device__ inline void pair_eval(const float4 &xt, const float4 &s, float off2, float &S0, float &S1,
                                  float &S2)
 {
     float dx = s.x - xt.x, dy = s.y - xt.y, dz = s.z - xt.z;
-    float den = fmaf(0.5f, dx * dx + dy * dy + dz * dz, off);
+    float den = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, off2)));
     float c = s.w * rcp32(den);
     S0 = fmaf(c, dx, S0);
     S1 = fmaf(c, dy, S1);
@@ -130,54 +130,6 @@ __device__ inline float4 to_frame(const float4 &s, int32_t rn, const double *__r
 __device__ inline int32_t src_node(int64_t j, int64_t p0, int p, const int32_t *__restrict__ recnode)
 {
     return j < p0 ? recnode[j] : (int32_t)((j - p0) / p);
-}
-
-// 3x3 solve in reference order (interpolation coordinates; not bit-exact work)
-__device__ inline void bary_solve(const double *T, const double *p, double *c)
-{
-    double a[9] = {T[0], T[1], T[2], T[3], T[4], T[5], T[6], T[7], T[8]};
-    int ip[3];
-    for (int j = 0; j < 3; j++) {
-        int jp = j;
-        double amax = fabs(a[j + 3 * j]);
-        for (int i = j + 1; i < 3; i++) {
-            double v = fabs(a[i + 3 * j]);
-            if (v > amax) {
-                amax = v;
-                jp = i;
-            }
-        }
-        ip[j] = jp;
-        if (jp != j)
-            for (int k = 0; k < 3; k++) {
-                double t = a[j + 3 * k];
-                a[j + 3 * k] = a[jp + 3 * k];
-                a[jp + 3 * k] = t;
-            }
-        double r = 1.0 / a[j + 3 * j];
-        for (int i = j + 1; i < 3; i++) a[i + 3 * j] *= r;
-        for (int k = j + 1; k < 3; k++) {
-            double t = -a[j + 3 * k];
-            for (int i = j + 1; i < 3; i++) a[i + 3 * k] = a[i + 3 * k] + a[i + 3 * j] * t;
-        }
-    }
-    c[0] = p[0];
-    c[1] = p[1];
-    c[2] = p[2];
-    for (int k = 0; k < 3; k++) {
-        int q = ip[k];
-        if (q != k) {
-            double t = c[k];
-            c[k] = c[q];
-            c[q] = t;
-        }
-    }
-    for (int k = 0; k < 3; k++)
-        for (int i = k + 1; i < 3; i++) c[i] = c[i] - c[k] * a[i + 3 * k];
-    for (int k = 2; k >= 0; k--) {
-        c[k] = c[k] / a[k + 3 * k];
-        for (int i = 0; i < k; i++) c[i] = c[i] - c[k] * a[i + 3 * k];
-    }
 }
 
 // monomials s^(i-j) t^j, index i(i+1)/2 + j (interp_utils.h:49-59)
@@ -233,21 +185,20 @@ __global__ void k_gather32(const double *__restrict__ x, const double *__restric
         int32_t id = recid[i];
         const double *c = centres + 4 * (int64_t)recnode[i];
         src[i] = make_float4((float)(x[id] - c[0]), (float)(y[id] - c[1]), (float)(z[id] - c[2]),
-                             (float)(vort[id] * area[id]));
+                             (float)(2.0 * vort[id] * area[id]));
     } else if (i < p0 + nprox) {
         double4 q = proxy[i - p0];
         const double *c = centres + 4 * ((i - p0) / p);
         src[i] = make_float4((float)(q.x - c[0]), (float)(q.y - c[1]), (float)(q.z - c[2]), 0.0f);
-        // For a particle x on the sphere, 1 - y.x = |x - y|^2/2 + (1 - |y|^2)/2.
+        // For a particle x on the sphere, 2 (1 - y.x) = |x - y|^2 + (1 - |y|^2).
         // Planar proxies (PARITY) are off the sphere and carry that offset.
         double nq = q.x * q.x + q.y * q.y + q.z * q.z;
-        aoff[i - p0] = planar ? (float)(0.5 * (1.0 - nq)) : 0.0f;
+        aoff[i - p0] = planar ? (float)(1.0 - nq) : 0.0f;
     }
 }
 
 // ------------------------------------------------------------------
 // upward pass (BLTC): modified charges q_k = sum_j L_k(b_j) w_j
-// one 256-thread block per charge node
 // ------------------------------------------------------------------
 // position (double) and weight of a particle record
 __device__ inline void rec_point(const double4 &r, int32_t, const double *, double *pt, double &w)
@@ -264,68 +215,127 @@ __device__ inline void rec_point(const float4 &r, int32_t rn, const double *__re
     pt[0] = c[0] + (double)r.x;
     pt[1] = c[1] + (double)r.y;
     pt[2] = c[2] + (double)r.z;
-    w = (double)r.w;
+    w = 0.5 * (double)r.w; // FP32 records carry 2 w
 }
 
+// barycentric coordinates of p from the node's inverse vertex matrix
+__device__ inline void bary_inv(const double *__restrict__ R, const double *p, double *c)
+{
+#pragma unroll
+    for (int m = 0; m < 3; m++) c[m] = fma(R[3 * m], p[0], fma(R[3 * m + 1], p[1], R[3 * m + 2] * p[2]));
+}
+
+// modified charges q_k = sum_idx Vinv[idx][k] M_idx from the moments M
 template <typename Real, int D>
-__global__ void __launch_bounds__(256) k_upward(const int32_t *__restrict__ charge_nodes,
-                                                const int4 *__restrict__ nrange,
-                                                const int32_t *__restrict__ recnode,
-                                                const double *__restrict__ centres,
-                                                const double *__restrict__ tri_xyz,
-                                                const double *__restrict__ vinv, int64_t p0,
-                                                typename Rec<Real>::T4 *src)
+__device__ inline void write_charges(const double *M, const double *__restrict__ vinv, int64_t p0, int32_t g,
+                                     typename Rec<Real>::T4 *src)
 {
     constexpr int P = npts<D>();
-    __shared__ double red[4][P];
-    int32_t g = charge_nodes[blockIdx.x];
-    const int4 nr = nrange[g];
-    const int32_t cnt = nr.y + nr.w;
-    const double *T = tri_xyz + 9 * (int64_t)g;
+    for (int k = 0; k < P; k++) {
+        double q = 0.0;
+#pragma unroll
+        for (int idx = 0; idx < P; idx++) q = fma(vinv[idx + k * P], M[idx], q); // vinv column-major
+        // weight slot of the proxy record (FP32 records carry 2 q)
+        src[p0 + (int64_t)g * P + k].w = (Real)(sizeof(Real) == 4 ? 2.0 * q : q);
+    }
+}
+
+// one wave per upward item (<= FSS_UP_CHUNK consecutive records of one
+// charge node, read coalesced): moments M_idx = sum_j m_idx(b_j) w_j with
+// normalised barycentric b_j; an item that covers its whole node writes the
+// charges, the others write partial moments
+template <typename Real, int D>
+__global__ void __launch_bounds__(64) k_upward(const int4 *__restrict__ items,
+                                               const int32_t *__restrict__ recnode,
+                                               const double *__restrict__ centres,
+                                               const double *__restrict__ tinv,
+                                               const double *__restrict__ vinv, int64_t p0,
+                                               typename Rec<Real>::T4 *src, double *partial)
+{
+    constexpr int P = npts<D>();
+    const int4 it = items[blockIdx.x];
+    const int lane = threadIdx.x;
+    const double *R = tinv + 9 * (int64_t)it.x;
+    double Rl[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) Rl[k] = R[k];
     double M[P];
 #pragma unroll
     for (int k = 0; k < P; k++) M[k] = 0.0;
-    for (int32_t j = threadIdx.x; j < cnt; j += 256) {
-        int32_t pos = j < nr.y ? nr.x + j : nr.z + (j - nr.y);
+    for (int32_t j = lane; j < it.z; j += 64) {
+        int32_t pos = it.y + j;
         double pt[3], c[3], m[P], w;
         rec_point(src[pos], recnode[pos], centres, pt, w);
-        bary_solve(T, pt, c);
-        double sg = c[0] + c[1] + c[2];
-        monomials<D>(c[0] / sg, c[1] / sg, m);
+        bary_inv(Rl, pt, c);
+        double is = 1.0 / (c[0] + c[1] + c[2]);
+        monomials<D>(c[0] * is, c[1] * is, m);
 #pragma unroll
         for (int k = 0; k < P; k++) M[k] = fma(m[k], w, M[k]);
     }
-    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < P; k++) {
-        double v = M[k];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        if (lane == 0) red[wv][k] = v;
+    for (int k = 0; k < P; k++)
+        for (int off = 32; off > 0; off >>= 1) M[k] += __shfl_xor(M[k], off, 64);
+    if (lane != 0) return;
+    if (it.w < 0) {
+        write_charges<Real, D>(M, vinv, p0, it.x, src);
+    } else {
+        double *o = partial + (int64_t)it.w * P;
+#pragma unroll
+        for (int k = 0; k < P; k++) o[k] = M[k];
     }
-    __syncthreads();
-    if (threadIdx.x < P) {
-        int k = threadIdx.x;
-        double q = 0.0;
-        for (int idx = 0; idx < P; idx++) {
-            double Mi = red[0][idx] + red[1][idx] + red[2][idx] + red[3][idx];
-            q = fma(vinv[idx + k * P], Mi, q); // vinv column-major: Vinv[idx][k]
-        }
-        int64_t pos = p0 + (int64_t)g * P + k;
-        src[pos].w = (Real)q; // weight slot of the proxy record
+}
+
+// nodes with several items: one wave per node sums the partial moments
+template <typename Real, int D>
+__global__ void __launch_bounds__(64) k_upward_finish(const int4 *__restrict__ multi,
+                                                      const double *__restrict__ partial,
+                                                      const double *__restrict__ vinv, int64_t p0,
+                                                      typename Rec<Real>::T4 *src)
+{
+    constexpr int P = npts<D>();
+    const int4 mn = multi[blockIdx.x];
+    const int lane = threadIdx.x;
+    double M[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) M[k] = 0.0;
+    for (int32_t j = mn.y + lane; j < mn.z; j += 64) {
+        const double *o = partial + (int64_t)j * P;
+#pragma unroll
+        for (int k = 0; k < P; k++) M[k] += o[k];
     }
+#pragma unroll
+    for (int k = 0; k < P; k++)
+        for (int off = 32; off > 0; off >>= 1) M[k] += __shfl_xor(M[k], off, 64);
+    if (lane == 0) write_charges<Real, D>(M, vinv, p0, mn.x, src);
 }
 
 // ------------------------------------------------------------------
 // staged source loop shared by the near and proxy kernels
 // ------------------------------------------------------------------
+// Evaluates the staged entries [0, fill) owned by this lane (k = ph, ph + P,
+// ...), two per iteration, without branches in the loop body.
+template <typename Eval>
+__device__ inline void run_chunk(int fill, int ph, int P, bool act, Eval eval)
+{
+    if (!act) return;
+    int k = ph;
+    for (; k + P < fill; k += 2 * P) {
+        eval(k);
+        eval(k + P);
+    }
+    if (k < fill) eval(k);
+}
+
 // Stages the sources of ranges [r0, r1) that are not flagged self into LDS in
-// chunks and calls body(k) for every staged entry k owned by this lane.
-// Records are moved to the target node's frame (to_frame) while staged.
-template <typename T4, typename Body>
+// chunks of NCH; after every full chunk the lane's entries are evaluated and
+// flush() moves the FP32 partial sums into FP64.  Records are moved to the
+// target node's frame (to_frame) while staged.  The caller evaluates the last
+// (partial) chunk.
+template <typename T4, typename Eval, typename Flush>
 __device__ inline void staged_ranges(const Range *__restrict__ ranges, int r0, int r1,
                                      const T4 *__restrict__ src, const int32_t *__restrict__ recnode,
                                      int64_t p0, int p, const double *__restrict__ centres, const double *ct,
-                                     T4 *lds, int ph, int P, bool act, Body body, int &fill)
+                                     T4 *lds, int ph, int P, bool act, Eval eval, Flush flush, int &fill)
 {
     const int lane = threadIdx.x;
     for (int r = r0; r < r1; r++) {
@@ -335,9 +345,8 @@ __device__ inline void staged_ranges(const Range *__restrict__ ranges, int r0, i
             int m = min(64, rg.count - base);
             if (fill + m > NCH) {
                 __syncthreads();
-                if (act)
-                    for (int k = ph; k < fill; k += P) body(k);
-                body(-1); // chunk boundary
+                run_chunk(fill, ph, P, act, eval);
+                flush();
                 __syncthreads();
                 fill = 0;
             }
@@ -366,6 +375,47 @@ __device__ inline void abs_pos(const float4 &, int32_t id, const double *__restr
     o[1] = y[id];
     o[2] = z[id];
 }
+
+// Per-lane accumulators of R targets: FP64 sums directly; FP32 sums partials
+// of at most one chunk per lane, flushed into FP64.
+template <typename Real, int R> struct Acc;
+template <int R> struct Acc<double, R> {
+    double S[R][3];
+    __device__ void zero()
+    {
+#pragma unroll
+        for (int q = 0; q < R; q++) S[q][0] = S[q][1] = S[q][2] = 0.0;
+    }
+    __device__ double *part(int q) { return S[q]; }
+    __device__ void flush() {}
+    __device__ double total(int q, int c) const { return S[q][c]; }
+};
+template <int R> struct Acc<float, R> {
+    double S[R][3];
+    float Pp[R][3];
+    __device__ void zero()
+    {
+#pragma unroll
+        for (int q = 0; q < R; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                S[q][c] = 0.0;
+                Pp[q][c] = 0.f;
+            }
+    }
+    __device__ float *part(int q) { return Pp[q]; }
+    __device__ void flush()
+    {
+#pragma unroll
+        for (int q = 0; q < R; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                S[q][c] += (double)Pp[q][c];
+                Pp[q][c] = 0.f;
+            }
+    }
+    __device__ double total(int q, int c) const { return S[q][c]; }
+};
 
 // ------------------------------------------------------------------
 // near field (class A): direct pairs, R targets per lane
@@ -403,36 +453,23 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
         xt[q] = to_frame(src[j], recnode[j], centres, ct);
         tid[q] = recid[j];
     }
-    double S[R][3];
-    Real Pp[R][3];
-#pragma unroll
-    for (int q = 0; q < R; q++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            S[q][c] = 0.0;
-            Pp[q][c] = (Real)0;
-        }
-    auto body = [&](int k) {
-        if (k < 0) { // chunk boundary: flush the Real partials into FP64
-#pragma unroll
-            for (int q = 0; q < R; q++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    S[q][c] += (double)Pp[q][c];
-                    Pp[q][c] = (Real)0;
-                }
-            return;
-        }
+    Acc<Real, R> A;
+    A.zero();
+    auto eval = [&](int k) {
         T4 s = lds[k];
 #pragma unroll
-        for (int q = 0; q < R; q++) pair_eval(xt[q], s, (Real)0, Pp[q][0], Pp[q][1], Pp[q][2]);
+        for (int q = 0; q < R; q++) {
+            Real *pq = A.part(q);
+            pair_eval(xt[q], s, (Real)0, pq[0], pq[1], pq[2]);
+        }
     };
+    auto flush = [&]() { A.flush(); };
     int fill = 0;
-    staged_ranges(ranges, it.a_beg, it.a_end, src, recnode, p0, p, centres, ct, lds, ph, P, act, body, fill);
+    staged_ranges(ranges, it.a_beg, it.a_end, src, recnode, p0, p, centres, ct, lds, ph, P, act, eval, flush,
+                  fill);
     __syncthreads();
-    if (act)
-        for (int k = ph; k < fill; k += P) body(k);
-    body(-1);
+    run_chunk(fill, ph, P, act, eval);
+    A.flush();
     // ranges that may hold the target itself: skip the self pair by particle id
     for (int r = it.a_beg; r < it.a_end; r++) {
         Range rg = ranges[r];
@@ -443,15 +480,17 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
                 T4 s = to_frame(src[j], recnode[j], centres, ct);
                 int32_t sid = recid[j];
 #pragma unroll
-                for (int q = 0; q < R; q++)
-                    if (sid != tid[q]) pair_eval(xt[q], s, (Real)0, Pp[q][0], Pp[q][1], Pp[q][2]);
+                for (int q = 0; q < R; q++) {
+                    Real *pq = A.part(q);
+                    if (sid != tid[q]) pair_eval(xt[q], s, (Real)0, pq[0], pq[1], pq[2]);
+                }
             }
+        A.flush();
     }
-    body(-1);
 #pragma unroll
     for (int q = 0; q < R; q++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) red[lane][3 * q + c] = S[q][c];
+        for (int c = 0; c < 3; c++) red[lane][3 * q + c] = A.total(q, c);
     __syncthreads();
     if (ph == 0) {
 #pragma unroll
@@ -508,42 +547,29 @@ __global__ void __launch_bounds__(64) k_proxy(const ProxyItem *__restrict__ item
         yk[q] = src[pk];
         at[q] = aoff ? (Real)aoff[pk - p0] : (Real)0;
     }
-    double S[R][3];
-    Real Pp[R][3];
-#pragma unroll
-    for (int q = 0; q < R; q++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            S[q][c] = 0.0;
-            Pp[q][c] = (Real)0;
-        }
-    auto body = [&](int k) {
-        if (k < 0) {
-#pragma unroll
-            for (int q = 0; q < R; q++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    S[q][c] += (double)Pp[q][c];
-                    Pp[q][c] = (Real)0;
-                }
-            return;
-        }
+    Acc<Real, R> A;
+    A.zero();
+    auto eval = [&](int k) {
         T4 s = lds[k];
 #pragma unroll
-        for (int q = 0; q < R; q++) pair_far(yk[q], s, at[q], Pp[q][0], Pp[q][1], Pp[q][2]);
+        for (int q = 0; q < R; q++) {
+            Real *pq = A.part(q);
+            pair_far(yk[q], s, at[q], pq[0], pq[1], pq[2]);
+        }
     };
+    auto flush = [&]() { A.flush(); };
     const double *cp = centres + 4 * (int64_t)it.node;
     const double ct[3] = {cp[0], cp[1], cp[2]};
     int fill = 0;
-    staged_ranges(ranges, it.b_beg, it.b_end, src, recnode, p0, PP, centres, ct, lds, ph, P, act, body, fill);
+    staged_ranges(ranges, it.b_beg, it.b_end, src, recnode, p0, PP, centres, ct, lds, ph, P, act, eval, flush,
+                  fill);
     __syncthreads();
-    if (act)
-        for (int k = ph; k < fill; k += P) body(k);
-    body(-1);
+    run_chunk(fill, ph, P, act, eval);
+    A.flush();
 #pragma unroll
     for (int q = 0; q < R; q++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) red[lane][3 * q + c] = S[q][c];
+        for (int c = 0; c < 3; c++) red[lane][3 * q + c] = A.total(q, c);
     __syncthreads();
     if (ph == 0) {
         double *o = phi + (int64_t)blockIdx.x * 3 * PP;
@@ -622,70 +648,103 @@ __global__ void k_solve(const ProxySlot *__restrict__ slots, int64_t nslot,
 // per-particle reduction over levels, interpolation of the proxy values of
 // every node on the particle's path, rhs_func scaling (rhs_eval.h:54-55)
 // ------------------------------------------------------------------
+// interpolant of node g (slot sl) at point pt, added to a
 template <int D>
-__global__ void k_combine(const int32_t *__restrict__ inv_active, const int32_t *__restrict__ nodeof,
-                          const int32_t *__restrict__ node_slot, const double *__restrict__ alpha,
-                          const double *__restrict__ tri_xyz, const double *__restrict__ x,
-                          const double *__restrict__ y, const double *__restrict__ z,
-                          const Instance *__restrict__ inst, int ninst, int64_t n, int64_t ostride,
-                          const double *__restrict__ out, int normalise, double scale, double omega,
-                          double *u, double *v, double *w, double *dvort)
+__device__ inline void add_interp(const double *__restrict__ tinv, const double *__restrict__ alpha, int32_t g,
+                                  int32_t sl, const double *pt, int normalise, double &a0, double &a1, double &a2)
 {
     constexpr int P = npts<D>();
+    double c[3], m[P];
+    bary_inv(tinv + 9 * (int64_t)g, pt, c);
+    double s = c[0], t = c[1];
+    if (normalise) {
+        double is = 1.0 / (c[0] + c[1] + c[2]);
+        s = c[0] * is;
+        t = c[1] * is;
+    }
+    monomials<D>(s, t, m);
+    const double *al = alpha + (int64_t)sl * 3 * P;
+    double b0 = 0, b1 = 0, b2 = 0;
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+        b0 = fma(m[q], al[q], b0);
+        b1 = fma(m[q], al[P + q], b1);
+        b2 = fma(m[q], al[2 * P + q], b2);
+    }
+    a0 += b0;
+    a1 += b1;
+    a2 += b2;
+}
+
+// One thread per main record, in tree order (neighbouring threads share
+// nodes, so node data is read once per wave): sums the particle's near-field
+// results over its main path (output slot of its node + its offset in the
+// node's range) and the interpolants of the nodes on that path; the rare
+// memberships outside the main path come from the per-level maps.  Then
+// the rhs_func scaling (rhs_eval.h:54-55).
+template <int D>
+__global__ void k_combine(const int32_t *__restrict__ recid, const int32_t *__restrict__ recnode,
+                          const uint8_t *__restrict__ rinfo, const int4 *__restrict__ nrange,
+                          const int32_t *__restrict__ oslot, const int32_t *__restrict__ node_slot,
+                          const int32_t *__restrict__ nodeof, const int32_t *__restrict__ inv_active,
+                          const double *__restrict__ alpha, const double *__restrict__ tinv,
+                          const double *__restrict__ x, const double *__restrict__ y,
+                          const double *__restrict__ z, const Instance *__restrict__ inst, int ninst,
+                          int64_t nrec, int64_t ostride, const double *__restrict__ out, int normalise,
+                          double scale, double omega, double *u, double *v, double *w, double *dvort)
+{
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    // instance of particle i: last k with pbase <= i
+    if (i >= nrec) return;
+    // instance of record i: last k with rbase <= i
     int lo = 0, hi = ninst - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) / 2;
-        if (inst[mid].pbase <= i) lo = mid;
+        if (inst[mid].rbase <= i) lo = mid;
         else hi = mid - 1;
     }
     const Instance I = inst[lo];
-    const int64_t li = i - I.pbase;
-    const int L = I.L;
-    double pt[3] = {x[i], y[i], z[i]};
+    if (i - I.rbase >= I.n) return; // exception record
+    const int64_t id = recid[i];
+    const double pt[3] = {x[id], y[id], z[id]};
+    const int info = rinfo[i];
+    const int e = info & 0x7f;
     double a0 = 0, a1 = 0, a2 = 0;
-    for (int l = 0; l < L; l++) {
-        int64_t at = I.loff + (int64_t)l * I.n + li;
-        int32_t pos = inv_active[at];
-        if (pos >= 0) {
-            a0 += out[pos];
-            a1 += out[ostride + pos];
-            a2 += out[2 * ostride + pos];
-        }
-        int32_t g = nodeof[at];
-        int32_t sl = g >= 0 ? node_slot[g] : -1;
-        if (sl >= 0) {
-            double c[3], m[P];
-            bary_solve(tri_xyz + 9 * (int64_t)g, pt, c);
-            double s = c[0], t = c[1];
-            if (normalise) {
-                double sg = c[0] + c[1] + c[2];
-                s = c[0] / sg;
-                t = c[1] / sg;
+    if (e > 0) {
+        const int ld = e - 1;
+        const int64_t locd = recnode[i] - I.nbase - fssg::level_offset(ld);
+        for (int l = 0; l < e; l++) {
+            const int32_t g = (int32_t)(I.nbase + fssg::level_offset(l) + (locd >> (2 * (ld - l))));
+            const int32_t os = oslot[g];
+            if (os >= 0) {
+                const int64_t pos = os + (i - nrange[g].x);
+                a0 += out[pos];
+                a1 += out[ostride + pos];
+                a2 += out[2 * ostride + pos];
             }
-            monomials<D>(s, t, m);
-            const double *al = alpha + (int64_t)sl * 3 * P;
-            double b0 = 0, b1 = 0, b2 = 0;
-#pragma unroll
-            for (int q = 0; q < P; q++) {
-                b0 = fma(m[q], al[q], b0);
-                b1 = fma(m[q], al[P + q], b1);
-                b2 = fma(m[q], al[2 * P + q], b2);
-            }
-            a0 += b0;
-            a1 += b1;
-            a2 += b2;
+            const int32_t sl = node_slot[g];
+            if (sl >= 0) add_interp<D>(tinv, alpha, g, sl, pt, normalise, a0, a1, a2);
         }
     }
-    a0 *= scale;
-    a1 *= scale;
-    a2 *= scale;
-    u[i] = a0;
-    v[i] = a1;
-    w[i] = a2;
-    dvort[i] = -2 * omega * a2;
+    if (info & 0x80) {
+        const int64_t li = id - I.pbase;
+        for (int l = e; l < I.L; l++) {
+            const int64_t at = I.loff + (int64_t)l * I.n + li;
+            const int32_t g = nodeof[at];
+            if (g < 0) continue;
+            const int32_t pos = inv_active[at];
+            if (pos >= 0) {
+                a0 += out[pos];
+                a1 += out[ostride + pos];
+                a2 += out[2 * ostride + pos];
+            }
+            const int32_t sl = node_slot[g];
+            if (sl >= 0) add_interp<D>(tinv, alpha, g, sl, pt, normalise, a0, a1, a2);
+        }
+    }
+    u[id] = a0 * scale;
+    v[id] = a1 * scale;
+    w[id] = a2 * scale;
+    dvort[id] = -2 * omega * (a2 * scale);
 }
 
 // ------------------------------------------------------------------
@@ -705,6 +764,7 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
     FSS_TRY(ctx->out3.ensure(sizeof(double) * 3 * ost));
     FSS_TRY(ctx->phi.ensure(sizeof(double) * 3 * p * std::max<int64_t>(1, T.n_proxy)));
     FSS_TRY(ctx->alpha.ensure(sizeof(double) * 3 * p * std::max<int64_t>(1, T.n_slot)));
+    FSS_TRY(ctx->upart.ensure(sizeof(double) * p * std::max<int64_t>(1, T.n_up_partial)));
     if (ev) FSS_HIP(hipEventRecord(ev[0], s));
     unsigned gs = (unsigned)div_up(nsrc, 256);
     if (!mixed) {
@@ -712,22 +772,28 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
         double4 *src = ctx->src64.as<double4>();
         hipLaunchKernelGGL(k_gather64, dim3(gs), dim3(256), 0, s, pp->x, pp->y, pp->z, pp->vort,
                            pp->area, T.recid.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, src);
-        if (bltc && T.n_charge > 0)
-            hipLaunchKernelGGL((k_upward<double, D>), dim3((unsigned)T.n_charge), dim3(256), 0, s,
-                               T.charge_nodes.as<int32_t>(), T.nrange.as<int4>(), T.recnode.as<int32_t>(),
-                               T.tri_info.as<double>(), T.tri_xyz.as<double>(), ctx->d_vinv.as<double>(),
-                               p0, src);
+        if (bltc && T.n_up > 0) {
+            hipLaunchKernelGGL((k_upward<double, D>), dim3((unsigned)T.n_up), dim3(64), 0, s,
+                               T.up_items.as<int4>(), T.recnode.as<int32_t>(), T.tri_info.as<double>(),
+                               T.tinv.as<double>(), ctx->d_vinv.as<double>(), p0, src,
+                               ctx->upart.as<double>());
+            if (T.n_up_multi > 0)
+                hipLaunchKernelGGL((k_upward_finish<double, D>), dim3((unsigned)T.n_up_multi), dim3(64), 0, s,
+                                   T.up_multi.as<int4>(), ctx->upart.as<double>(), ctx->d_vinv.as<double>(), p0,
+                                   src);
+        }
         if (T.n_proxy > 0)
             hipLaunchKernelGGL((k_proxy<double, D>), dim3((unsigned)T.n_proxy), dim3(64), 0, s,
                                T.proxy_items.as<ProxyItem>(), T.ranges_b.as<Range>(), src,
                                T.recnode.as<int32_t>(), (const float *)nullptr, p0, T.tri_info.as<double>(),
                                T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
         if (ev) FSS_HIP(hipEventRecord(ev[1], s));
-        if (T.n_near > 0)
-            hipLaunchKernelGGL((k_near<double, 2>), dim3((unsigned)T.n_near), dim3(64), 0, s,
-                               T.near_items.as<NearItem>(), T.ranges_a.as<Range>(), src,
-                               T.recid.as<int32_t>(), T.recnode.as<int32_t>(), p0, p, T.tri_info.as<double>(),
-                               pp->x, pp->y, pp->z, ost, ctx->out3.as<double>());
+        if (T.n_near > 0) {
+            auto kn = ctx->near_r == 2 ? k_near<double, 2> : k_near<double, 4>; // default 4
+            hipLaunchKernelGGL(kn, dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
+                               T.ranges_a.as<Range>(), src, T.recid.as<int32_t>(), T.recnode.as<int32_t>(), p0, p,
+                               T.tri_info.as<double>(), pp->x, pp->y, pp->z, ost, ctx->out3.as<double>());
+        }
         if (ev) FSS_HIP(hipEventRecord(ev[2], s));
     } else {
         FSS_TRY(ctx->src32.ensure(sizeof(float4) * nsrc + sizeof(float) * nprox));
@@ -737,33 +803,40 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
                            pp->area, T.recid.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, p,
                            T.recnode.as<int32_t>(), T.tri_info.as<double>(), bltc ? 0 : 1, src,
                            aoff);
-        if (bltc && T.n_charge > 0)
-            hipLaunchKernelGGL((k_upward<float, D>), dim3((unsigned)T.n_charge), dim3(256), 0, s,
-                               T.charge_nodes.as<int32_t>(), T.nrange.as<int4>(), T.recnode.as<int32_t>(),
-                               T.tri_info.as<double>(), T.tri_xyz.as<double>(), ctx->d_vinv.as<double>(),
-                               p0, src);
+        if (bltc && T.n_up > 0) {
+            hipLaunchKernelGGL((k_upward<float, D>), dim3((unsigned)T.n_up), dim3(64), 0, s,
+                               T.up_items.as<int4>(), T.recnode.as<int32_t>(), T.tri_info.as<double>(),
+                               T.tinv.as<double>(), ctx->d_vinv.as<double>(), p0, src,
+                               ctx->upart.as<double>());
+            if (T.n_up_multi > 0)
+                hipLaunchKernelGGL((k_upward_finish<float, D>), dim3((unsigned)T.n_up_multi), dim3(64), 0, s,
+                                   T.up_multi.as<int4>(), ctx->upart.as<double>(), ctx->d_vinv.as<double>(), p0,
+                                   src);
+        }
         if (T.n_proxy > 0)
             hipLaunchKernelGGL((k_proxy<float, D>), dim3((unsigned)T.n_proxy), dim3(64), 0, s,
                                T.proxy_items.as<ProxyItem>(), T.ranges_b.as<Range>(), src,
                                T.recnode.as<int32_t>(), (const float *)aoff, p0, T.tri_info.as<double>(),
                                T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
         if (ev) FSS_HIP(hipEventRecord(ev[1], s));
-        if (T.n_near > 0)
-            hipLaunchKernelGGL((k_near<float, 2>), dim3((unsigned)T.n_near), dim3(64), 0, s,
-                               T.near_items.as<NearItem>(), T.ranges_a.as<Range>(), src,
-                               T.recid.as<int32_t>(), T.recnode.as<int32_t>(), p0, p, T.tri_info.as<double>(),
-                               pp->x, pp->y, pp->z, ost, ctx->out3.as<double>());
+        if (T.n_near > 0) {
+            auto kn = ctx->near_r == 4 ? k_near<float, 4> : k_near<float, 2>; // default 2
+            hipLaunchKernelGGL(kn, dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
+                               T.ranges_a.as<Range>(), src, T.recid.as<int32_t>(), T.recnode.as<int32_t>(), p0, p,
+                               T.tri_info.as<double>(), pp->x, pp->y, pp->z, ost, ctx->out3.as<double>());
+        }
         if (ev) FSS_HIP(hipEventRecord(ev[2], s));
     }
     if (T.n_slot > 0)
         hipLaunchKernelGGL((k_solve<D>), dim3((unsigned)div_up(3 * T.n_slot, 128)), dim3(128), 0, s,
                            T.proxy_slots.as<ProxySlot>(), T.n_slot, ctx->phi.as<double>(),
                            ctx->d_lu.as<double>(), ctx->d_ipiv.as<int32_t>(), ctx->alpha.as<double>());
-    hipLaunchKernelGGL((k_combine<D>), dim3((unsigned)div_up(T.n, 256)), dim3(256), 0, s,
-                       T.inv_active.as<int32_t>(), T.nodeof.as<int32_t>(), T.node_slot.as<int32_t>(),
-                       ctx->alpha.as<double>(), T.tri_xyz.as<double>(), pp->x, pp->y, pp->z,
-                       T.inst.as<Instance>(), T.ninst, T.n, ost, ctx->out3.as<double>(), bltc ? 1 : 0,
-                       scale, ctx->cfg.omega, out->u, out->v, out->w, out->dvort);
+    hipLaunchKernelGGL((k_combine<D>), dim3((unsigned)div_up(p0, 256)), dim3(256), 0, s, T.recid.as<int32_t>(),
+                       T.recnode.as<int32_t>(), T.rinfo.as<uint8_t>(), T.nrange.as<int4>(), T.oslot.as<int32_t>(),
+                       T.node_slot.as<int32_t>(), T.nodeof.as<int32_t>(), T.inv_active.as<int32_t>(),
+                       ctx->alpha.as<double>(), T.tinv.as<double>(), pp->x, pp->y, pp->z, T.inst.as<Instance>(),
+                       T.ninst, p0, ost, ctx->out3.as<double>(), bltc ? 1 : 0, scale, ctx->cfg.omega, out->u,
+                       out->v, out->w, out->dvort);
     if (ev) FSS_HIP(hipEventRecord(ev[3], s));
     FSS_HIP(hipGetLastError());
     return FSS_OK;

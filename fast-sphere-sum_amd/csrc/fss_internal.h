@@ -120,6 +120,8 @@ struct ProxySlot {
 #define FSS_SELF_BIT (1 << 30)
 // Sources per proxy work item before a node's class-B list is split.
 #define FSS_PROXY_CHUNK 4096
+// Records per (one-wave) item of the upward pass.
+#define FSS_UP_CHUNK 512
 
 // One problem instance of a (possibly batched) tree: particles
 // [pbase, pbase + n) of the concatenated state, L levels, and the offset of its
@@ -130,6 +132,8 @@ struct Instance {
     int64_t loff;
     int32_t L;
     int32_t pad;
+    int64_t rbase; // first source record of the instance (main records first)
+    int64_t nbase; // first global node of the instance
 };
 
 // Interpolation constants (host copy + device copy).
@@ -165,6 +169,8 @@ struct Tree {
     DevBuf inter;    // int32 [7][ninter] AoS
     DevBuf recid;    // int32 [p0]: particle of every record
     DevBuf recnode;  // int32 [p0]: node whose centre is the origin of the record (mixed precision)
+    DevBuf rinfo;    // uint8 [p0]: main records: levels on the main path | 0x80 if exception members exist
+    DevBuf oslot;    // int32 [nn]: output slot of a node's first main target, or -1 (no near work)
     DevBuf nrange;   // int4 [nn]: {main start, main count, exception start, exception count}
     DevBuf inv_active; // int32 [L][n]: output slot of a particle in a class-A node, or -1
     DevBuf nodeof;     // int32 [L][n]: global node of a particle per level, or -1 (dropped)
@@ -181,6 +187,12 @@ struct Tree {
     int ninst = 1;
     bool batched = false;
     DevBuf charge_nodes; // int32 [n_charge] nodes whose proxies carry charges (BLTC)
+    // upward pass (BLTC): items of <= FSS_UP_CHUNK records of one charge node;
+    // nodes with several items sum their partial moments in a second pass
+    int64_t n_up = 0, n_up_multi = 0, n_up_partial = 0;
+    DevBuf up_items;    // int4 [n_up] {node, first record, records, partial index or -1}
+    DevBuf up_multi;    // int4 [n_up_multi] {node, first partial, end partial, 0}
+    DevBuf tinv;        // double [9 nn]: rows r_m with barycentric c_m = r_m . p
     DevBuf proxy_xyz;   // double4 [nn * p] static proxy positions (w unused)
     int64_t pairs_near = 0, pairs_far = 0;
 };
@@ -240,6 +252,8 @@ struct fss_ctx {
     fss::DevBuf out3;    // 3 doubles SoA [p0] per-position partial velocities
     fss::DevBuf phi;     // 3*p doubles per proxy work item (partial proxy values)
     fss::DevBuf alpha;   // 3*p doubles per interpolant slot
+    fss::DevBuf upart;   // p doubles per partial moment of the upward pass
+    int near_r = 0;      // targets per lane of the near kernel: 2, 4 or 0 (4 in FP64, 2 in FP32)
     fss::DevBuf rk_k;    // 4 stages x 4 fields x n
     fss::DevBuf rk_y;    // stage state 4 x n
     int64_t rk_n = 0;
@@ -258,6 +272,9 @@ namespace fss {
 
 // ---- kernels / launchers (fss_tree.hip) ----
 fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const double *z, int64_t n);
+// derived per-node data shared by single and batched plans (inverse triangle
+// matrices, upward-pass items); called last by both builders
+fss_status plan_finish(fss_ctx *ctx);
 fss_status tree_export(fss_ctx *ctx, int32_t *loc, uint8_t *found, int32_t *counts,
                        int32_t *offsets, int32_t *ids, int32_t *inter, double *tri_info,
                        double *tri_xyz);

@@ -126,14 +126,32 @@ __global__ void k_locate(const double *__restrict__ x, const double *__restrict_
 }
 
 __global__ void k_level_keys(const int32_t *loc, const uint8_t *found, int64_t n, uint32_t sentinel,
-                             uint32_t *keys, int32_t *vals, int32_t *counts_level)
+                             uint32_t *keys, int32_t *vals)
 {
     int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
     bool f = found[id] != 0;
     keys[id] = f ? (uint32_t)loc[id] : sentinel;
     vals[id] = (int32_t)id;
-    if (f) atomicAdd(&counts_level[loc[id]], 1);
+}
+
+// list lengths from the sorted keys of one level: segment starts, then
+// lengths at the segment ends
+__global__ void k_seg_start(const uint32_t *__restrict__ keys, int64_t n, uint32_t sentinel, int32_t *start)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = keys[i];
+    if (k != sentinel && (i == 0 || keys[i - 1] != k)) start[k] = (int32_t)i;
+}
+
+__global__ void k_seg_count(const uint32_t *__restrict__ keys, int64_t n, uint32_t sentinel,
+                            const int32_t *__restrict__ start, int32_t *counts_level)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = keys[i];
+    if (k != sentinel && (i == n - 1 || keys[i + 1] != k)) counts_level[k] = (int32_t)(i + 1 - start[k]);
 }
 
 // ------------------------------------------------------------------
@@ -143,7 +161,7 @@ __global__ void k_level_keys(const int32_t *loc, const uint8_t *found, int64_t n
 // is not on a list.  Its key holds one digit per level: the level-0 node
 // (0..19, 20 = not found), then the child digit (0..3) while covered, 4 after.
 __global__ void k_main_keys(const int32_t *__restrict__ loc, const uint8_t *__restrict__ found, int64_t n,
-                            int L, unsigned long long *keys, int32_t *vals, int32_t *ncnt, int32_t *ecnt,
+                            int L, unsigned long long *keys, int32_t *vals, int32_t *ecnt,
                             unsigned long long *nexc)
 {
     int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -159,10 +177,8 @@ __global__ void k_main_keys(const int32_t *__restrict__ loc, const uint8_t *__re
         if (l == 0) digit = cov ? lc : 20;
         else digit = cov ? lc - 4 * prev : 4;
         key = key * (l == 0 ? 1ull : 5ull) + (unsigned long long)digit;
-        int64_t g = level_offset(l) + lc;
-        if (cov) atomicAdd(&ncnt[g], 1);
-        else if (f) {
-            atomicAdd(&ecnt[g], 1);
+        if (!cov && f) { // exception membership (rare)
+            atomicAdd(&ecnt[level_offset(l) + lc], 1);
             atomicAdd(nexc, 1ull);
         }
         prev = lc;
@@ -175,7 +191,7 @@ __global__ void k_main_keys(const int32_t *__restrict__ loc, const uint8_t *__re
 // path) and particle of every main record
 __global__ void k_main_ranges(const int32_t *__restrict__ perm, const int32_t *__restrict__ loc,
                               const uint8_t *__restrict__ found, int64_t n, int L, int4 *nrange,
-                              int32_t *recid, int32_t *recnode)
+                              int32_t *recid, int32_t *recnode, uint8_t *rinfo)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -192,8 +208,13 @@ __global__ void k_main_ranges(const int32_t *__restrict__ perm, const int32_t *_
         if (!pcov || loc[(int64_t)l * n + pv] != lc) nrange[g].x = (int32_t)i;
         deepest = (int32_t)g;
     }
+    int e = 0; // levels on the main path
+    while (e < L && found[(int64_t)e * n + id]) e++;
+    bool exc = false;
+    for (int l = e + 1; l < L; l++) exc = exc || found[(int64_t)l * n + id];
     recid[i] = (int32_t)id;
     recnode[i] = deepest;
+    rinfo[i] = (uint8_t)(e | (exc ? 0x80 : 0));
 }
 
 // exception memberships: (node, particle) of every level a particle is on a
@@ -227,12 +248,13 @@ __global__ void k_exc_ranges(const unsigned long long *__restrict__ keys, const 
     recnode[base + q] = (int32_t)g;
 }
 
-__global__ void k_nrange_counts(const int32_t *__restrict__ ncnt, const int32_t *__restrict__ ecnt,
+// main members = list length - exception members
+__global__ void k_nrange_counts(const int32_t *__restrict__ counts, const int32_t *__restrict__ ecnt,
                                 int64_t nn, int4 *nrange)
 {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nn) return;
-    nrange[g].y = ncnt[g];
+    nrange[g].y = counts[g] - ecnt[g];
     nrange[g].w = ecnt[g];
 }
 
@@ -260,6 +282,25 @@ __global__ void k_trav_classify(const int4 *__restrict__ F, int64_t nF,
                                 unsigned long long *margin_bits)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // MAC margin: minimum over the wave first, one atomic per wave
+    unsigned long long mb = ~0ull;
+    if (i < nF) {
+        int4 q = F[i];
+        int ct = q.x, cs = q.y, lt = q.z, ls = q.w;
+        int32_t nt = node_count(counts, lt, ct), ns = node_count(counts, ls, cs);
+        const double *ti = tri_info + 4 * (level_offset(lt) + ct);
+        const double *si = tri_info + 4 * (level_offset(ls) + cs);
+        if (nt != 0 && ns != 0) {
+            double distance = fssg::great_circ_dist(ti, si, radius);
+            double separation = (ti[3] + si[3]) / distance;
+            if (distance > 0) mb = (unsigned long long)__double_as_longlong(fabs(separation - theta));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(mb, off, 64);
+        mb = o < mb ? o : mb;
+    }
+    if ((threadIdx.x & 63) == 0 && mb != ~0ull) atomicMin(margin_bits, mb);
     if (i >= nF) return;
     int4 q = F[i];
     int ct = q.x, cs = q.y, lt = q.z, ls = q.w;
@@ -270,11 +311,6 @@ __global__ void k_trav_classify(const int4 *__restrict__ F, int64_t nF,
         const double *si = tri_info + 4 * (level_offset(ls) + cs);
         double distance = fssg::great_circ_dist(ti, si, radius);
         double separation = (ti[3] + si[3]) / distance;
-        if (distance > 0) {
-            double mg = fabs(separation - theta);
-            unsigned long long b = (unsigned long long)__double_as_longlong(mg);
-            atomicMin(margin_bits, b);
-        }
         if ((distance > 0) && (separation < theta)) {
             a = (int8_t)((nt > many_count ? 2 : 0) + (ns > many_count ? 1 : 0));
         } else if ((nt < many_count) && (ns < many_count)) {
@@ -334,14 +370,18 @@ __global__ void k_plan_keys(const int32_t *__restrict__ inter, int64_t m, uint32
                             uint32_t *keys, int32_t *vals, unsigned long long *nA)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const int32_t *o = inter + 7 * i;
-    int type = o[6];
-    uint32_t cls = (type == 0 || type == 1) ? 0u : 1u;
-    uint32_t g = (uint32_t)(level_offset(o[0]) + o[2]);
-    keys[i] = cls * nn + g;
-    vals[i] = (int32_t)i;
-    if (cls == 0) atomicAdd(nA, 1ull);
+    uint32_t cls = 1u;
+    if (i < m) {
+        const int32_t *o = inter + 7 * i;
+        int type = o[6];
+        cls = (type == 0 || type == 1) ? 0u : 1u;
+        uint32_t g = (uint32_t)(level_offset(o[0]) + o[2]);
+        keys[i] = cls * nn + g;
+        vals[i] = (int32_t)i;
+    }
+    // class-A count: one atomic per wave
+    unsigned long long b = __ballot(cls == 0u);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(nA, (unsigned long long)__popcll(b));
 }
 
 // ranges per sorted interaction: 1, or 2 for particle sources whose node has
@@ -364,16 +404,32 @@ __global__ void k_plan_ranges(const int32_t *__restrict__ inter, const int32_t *
                               Range *ranges, uint32_t *rkeys, unsigned long long *pair_acc)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // pair accounting: near = direct-target evaluations, far = proxy-target
+    // (summed over the wave first: one atomic per wave and class)
+    unsigned long long accA = 0, accB = 0;
+    if (k < m) {
+        const int32_t *o = inter + 7 * sorted_idx[k];
+        int type = o[6];
+        bool proxy_src = bltc && (type == 1 || type == 3);
+        bool clsA = (type == 0 || type == 1);
+        unsigned long long tgt = clsA ? (unsigned long long)o[4] : (unsigned long long)p;
+        unsigned long long srcn = proxy_src ? (unsigned long long)p : (unsigned long long)o[5];
+        (clsA ? accA : accB) = tgt * srcn;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        accA += __shfl_xor(accA, off, 64);
+        accB += __shfl_xor(accB, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (accA) atomicAdd(&pair_acc[0], accA);
+        if (accB) atomicAdd(&pair_acc[1], accB);
+    }
     if (k >= m) return;
     const int32_t *o = inter + 7 * sorted_idx[k];
     int type = o[6];
     int64_t gs = level_offset(o[1]) + o[3];
     bool proxy_src = bltc && (type == 1 || type == 3);
     bool clsA = (type == 0 || type == 1);
-    // pair accounting: near = direct-target evaluations, far = proxy-target
-    unsigned long long tgt = clsA ? (unsigned long long)o[4] : (unsigned long long)p;
-    unsigned long long srcn = proxy_src ? (unsigned long long)p : (unsigned long long)o[5];
-    atomicAdd(&pair_acc[clsA ? 0 : 1], tgt * srcn);
     // particle sources of a node that contains the target node: the self pair
     // may lie in these ranges
     int lt = o[0], ls = o[1];
@@ -410,6 +466,15 @@ __global__ void k_plan_segments(const uint32_t *__restrict__ keys, int64_t m, ui
     uint32_t key = keys[k];
     if (k == 0 || keys[k - 1] != key) beg[key] = (int32_t)k;
     if (k == m - 1 || keys[k + 1] != key) end[key] = (int32_t)(k + 1);
+}
+
+// output slot of a node's first main target, -1 without near work
+__global__ void k_plan_oslot(const int32_t *__restrict__ nout, const int32_t *__restrict__ opos, int64_t nn,
+                             int32_t *oslot)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nn) return;
+    oslot[g] = nout[g] > 0 ? opos[g] : -1;
 }
 
 // per node: number of near chunks (main and exception targets), output
@@ -505,6 +570,27 @@ __global__ void k_plan_compact_flags(const uint8_t *flag, const int32_t *pos, ui
     if (flag[g]) list[pos[g]] = (int32_t)g;
 }
 
+// inverse of the vertex matrix [v1 v2 v3] of every node: row m is
+// (v_{m+1} x v_{m+2}) / det, so that the barycentric coordinates of p are
+// c_m = row_m . p
+__global__ void k_tinv(const double *__restrict__ tri_xyz, int64_t nn, double *tinv)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nn) return;
+    const double *T = tri_xyz + 9 * g;
+    double r[3][3];
+    for (int m = 0; m < 3; m++) {
+        const double *a = T + 3 * ((m + 1) % 3), *b = T + 3 * ((m + 2) % 3);
+        r[m][0] = a[1] * b[2] - a[2] * b[1];
+        r[m][1] = a[2] * b[0] - a[0] * b[2];
+        r[m][2] = a[0] * b[1] - a[1] * b[0];
+    }
+    double det = T[0] * r[0][0] + T[1] * r[0][1] + T[2] * r[0][2];
+    double id = 1.0 / det;
+    for (int m = 0; m < 3; m++)
+        for (int k = 0; k < 3; k++) tinv[9 * g + 3 * m + k] = r[m][k] * id;
+}
+
 // static proxy positions of every node: planar u v1 + v v2 + (1-u-v) v3
 // (fast_rhs.h:188-200) in PARITY, its projection onto the sphere in BLTC.
 __global__ void k_proxy_xyz(const double *__restrict__ tri_xyz, const double *__restrict__ pts,
@@ -595,7 +681,8 @@ void tree_release(Tree &t)
                       &t.inter, &t.inv_active, &t.nodeof, &t.node_slot, &t.proxy_slots, &t.recid,
                       &t.recnode, &t.nrange,
                       &t.near_items, &t.proxy_items, &t.ranges_a,
-                      &t.ranges_b, &t.charge_nodes, &t.proxy_xyz};
+                      &t.ranges_b, &t.charge_nodes, &t.proxy_xyz, &t.up_items, &t.up_multi, &t.tinv,
+                      &t.rinfo, &t.oslot};
     for (DevBuf *b : bufs) b->release();
     t.built = false;
 }
@@ -660,11 +747,15 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
     for (int l = 0; l < L; l++) {
         hipLaunchKernelGGL(k_level_keys, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s,
                            T.loc.as<int32_t>() + (int64_t)l * n, T.found.as<uint8_t>() + (int64_t)l * n,
-                           n, (uint32_t)level_size(l), S.a.as<uint32_t>(), S.c.as<int32_t>(),
-                           T.counts.as<int32_t>() + level_offset(l));
+                           n, (uint32_t)level_size(l), S.a.as<uint32_t>(), S.c.as<int32_t>());
         FSS_TRY(cub_sort_pairs(S.cub, S.a.as<uint32_t>(), S.b.as<uint32_t>(), S.c.as<int32_t>(),
                                S.d.as<int32_t>() + (int64_t)l * n, n, bits_for((uint64_t)level_size(l)),
                                s));
+        FSS_TRY(S.e.ensure(sizeof(int32_t) * level_size(l)));
+        hipLaunchKernelGGL(k_seg_start, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, S.b.as<uint32_t>(), n,
+                           (uint32_t)level_size(l), S.e.as<int32_t>());
+        hipLaunchKernelGGL(k_seg_count, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, S.b.as<uint32_t>(), n,
+                           (uint32_t)level_size(l), S.e.as<int32_t>(), T.counts.as<int32_t>() + level_offset(l));
     }
     FSS_TRY(cub_exclusive_sum(S.cub, T.counts.as<int32_t>(), T.offsets.as<int32_t>(), T.nn, s));
     // level bases: offsets at the first node of each level, plus the total
@@ -695,14 +786,14 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
         FSS_TRY(k1.ensure(sizeof(unsigned long long) * n));
         FSS_TRY(v0.ensure(sizeof(int32_t) * n));
         FSS_TRY(v1.ensure(sizeof(int32_t) * n));
-        FSS_TRY(cnt2.ensure(sizeof(int32_t) * 2 * T.nn));
+        FSS_TRY(cnt2.ensure(sizeof(int32_t) * T.nn));
         FSS_TRY(nexc.ensure(sizeof(unsigned long long)));
-        FSS_HIP(hipMemsetAsync(cnt2.p, 0, sizeof(int32_t) * 2 * T.nn, s));
+        FSS_HIP(hipMemsetAsync(cnt2.p, 0, sizeof(int32_t) * T.nn, s));
         FSS_HIP(hipMemsetAsync(nexc.p, 0, sizeof(unsigned long long), s));
-        int32_t *ncnt = cnt2.as<int32_t>(), *ecnt = ncnt + T.nn;
+        int32_t *ecnt = cnt2.as<int32_t>();
         unsigned gp = (unsigned)div_up(n, 256);
         hipLaunchKernelGGL(k_main_keys, dim3(gp), dim3(256), 0, s, T.loc.as<int32_t>(), T.found.as<uint8_t>(),
-                           n, L, k0.as<unsigned long long>(), v0.as<int32_t>(), ncnt, ecnt,
+                           n, L, k0.as<unsigned long long>(), v0.as<int32_t>(), ecnt,
                            nexc.as<unsigned long long>());
         uint64_t maxkey = 21;
         for (int l = 1; l < L; l++) maxkey *= 5;
@@ -714,13 +805,15 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
         if (T.p0 + T.nn * p > INT32_MAX) return set_error("tree: source records exceed 32-bit positions"), FSS_ERR_INVALID;
         FSS_TRY(T.recid.ensure(sizeof(int32_t) * T.p0));
         FSS_TRY(T.recnode.ensure(sizeof(int32_t) * T.p0));
+        FSS_TRY(T.rinfo.ensure(T.p0));
+        FSS_HIP(hipMemsetAsync(T.rinfo.p, 0, T.p0, s));
         FSS_TRY(T.nrange.ensure(sizeof(int4) * T.nn));
         FSS_HIP(hipMemsetAsync(T.nrange.p, 0, sizeof(int4) * T.nn, s));
-        hipLaunchKernelGGL(k_nrange_counts, dim3((unsigned)div_up(T.nn, 256)), dim3(256), 0, s, ncnt, ecnt, T.nn,
-                           T.nrange.as<int4>());
+        hipLaunchKernelGGL(k_nrange_counts, dim3((unsigned)div_up(T.nn, 256)), dim3(256), 0, s,
+                           T.counts.as<int32_t>(), ecnt, T.nn, T.nrange.as<int4>());
         hipLaunchKernelGGL(k_main_ranges, dim3(gp), dim3(256), 0, s, v1.as<int32_t>(), T.loc.as<int32_t>(),
                            T.found.as<uint8_t>(), n, L, T.nrange.as<int4>(), T.recid.as<int32_t>(),
-                           T.recnode.as<int32_t>());
+                           T.recnode.as<int32_t>(), T.rinfo.as<uint8_t>());
         if (ne > 0) {
             // exception memberships, sorted by (node, particle)
             DevBuf ek0, ek1, ev0, ev1;
@@ -904,6 +997,9 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
         T.n_slot = (int64_t)t6[2] + t6[3];
         T.nslot_out = (int64_t)t6[4] + t6[5];
     }
+    FSS_TRY(T.oslot.ensure(sizeof(int32_t) * T.nn));
+    hipLaunchKernelGGL(k_plan_oslot, dim3(gn), dim3(256), 0, s, nout.as<int32_t>(), opos.as<int32_t>(), T.nn,
+                       T.oslot.as<int32_t>());
     FSS_TRY(T.near_items.ensure(sizeof(NearItem) * std::max<int64_t>(1, T.n_near)));
     DevBuf prox_one;
     FSS_TRY(prox_one.ensure(sizeof(ProxyItem) * std::max<int64_t>(1, T.n_slot)));
@@ -984,7 +1080,7 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
         FSS_HIP(hipStreamSynchronize(s));
     }
     {
-        Instance one{0, n, 0, L, 0};
+        Instance one{0, n, 0, L, 0, 0, 0};
         FSS_TRY(T.inst.ensure(sizeof(Instance)));
         FSS_HIP(hipMemcpyAsync(T.inst.p, &one, sizeof(Instance), hipMemcpyHostToDevice, s));
         T.ninst = 1;
@@ -992,7 +1088,50 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
     }
     FSS_HIP(hipGetLastError());
     FSS_HIP(hipStreamSynchronize(s));
+    FSS_TRY(plan_finish(ctx));
     T.built = true;
+    return FSS_OK;
+}
+
+fss_status plan_finish(fss_ctx *ctx)
+{
+    Tree &T = ctx->tree;
+    hipStream_t s = ctx->stream;
+    FSS_TRY(T.tinv.ensure(sizeof(double) * 9 * std::max<int64_t>(1, T.nn)));
+    hipLaunchKernelGGL(k_tinv, dim3((unsigned)div_up(T.nn, 256)), dim3(256), 0, s, T.tri_xyz.as<double>(), T.nn,
+                       T.tinv.as<double>());
+    FSS_HIP(hipGetLastError());
+    // upward items: each charge node's records (main range, then exception
+    // range) in chunks of FSS_UP_CHUNK
+    std::vector<int32_t> charge(T.n_charge);
+    std::vector<int4> nr(T.nn);
+    if (T.n_charge > 0) FSS_TRY(d2h(charge.data(), T.charge_nodes.p, sizeof(int32_t) * T.n_charge, s));
+    if (T.n_charge > 0) FSS_TRY(d2h(nr.data(), T.nrange.p, sizeof(int4) * T.nn, s));
+    std::vector<int4> items, multi;
+    int32_t npart = 0;
+    for (int32_t g : charge) {
+        const int4 r = nr[g];
+        int32_t nchunk = (r.y + FSS_UP_CHUNK - 1) / FSS_UP_CHUNK + (r.w + FSS_UP_CHUNK - 1) / FSS_UP_CHUNK;
+        bool several = nchunk > 1;
+        int32_t first = npart;
+        for (int part = 0; part < 2; part++) {
+            int32_t start = part ? r.z : r.x, cnt = part ? r.w : r.y;
+            for (int32_t o = 0; o < cnt; o += FSS_UP_CHUNK)
+                items.push_back(make_int4(g, start + o, std::min(FSS_UP_CHUNK, cnt - o), several ? npart++ : -1));
+        }
+        if (nchunk == 0) items.push_back(make_int4(g, 0, 0, -1)); // empty node: zero charges
+        if (several) multi.push_back(make_int4(g, first, npart, 0));
+    }
+    T.n_up = (int64_t)items.size();
+    T.n_up_multi = (int64_t)multi.size();
+    T.n_up_partial = npart;
+    FSS_TRY(T.up_items.ensure(sizeof(int4) * std::max<int64_t>(1, T.n_up)));
+    FSS_TRY(T.up_multi.ensure(sizeof(int4) * std::max<int64_t>(1, T.n_up_multi)));
+    if (T.n_up > 0)
+        FSS_HIP(hipMemcpyAsync(T.up_items.p, items.data(), sizeof(int4) * T.n_up, hipMemcpyHostToDevice, s));
+    if (T.n_up_multi > 0)
+        FSS_HIP(hipMemcpyAsync(T.up_multi.p, multi.data(), sizeof(int4) * T.n_up_multi, hipMemcpyHostToDevice, s));
+    FSS_HIP(hipStreamSynchronize(s));
     return FSS_OK;
 }
 

@@ -56,7 +56,9 @@ class TreeStats(C.Structure):
     _fields_ = [("n", C.c_int64), ("tree_levels", C.c_int), ("n_nodes", C.c_int64),
                 ("n_interactions", C.c_int64), ("n_type", C.c_int64 * 4),
                 ("dropped", C.c_int64 * 32), ("pairs_near", C.c_int64), ("pairs_far", C.c_int64),
-                ("mac_margin", C.c_double), ("n_generations", C.c_int)]
+                ("mac_margin", C.c_double), ("n_generations", C.c_int),
+                ("n_records", C.c_int64), ("n_near_items", C.c_int64), ("n_proxy_items", C.c_int64),
+                ("n_interpolants", C.c_int64), ("n_charge_nodes", C.c_int64)]
 
 
 # exported symbol -> (restype, argtypes); the list is the ABI of include/fss.h
@@ -75,6 +77,7 @@ SIGNATURES = {
     "fss_batch_build": (_int, [_vp, _int, _vp, _vp, _vp, _vp, _vp]),
     "fss_autotune_levels": (_int, [_vp, C.POINTER(Particles), _int, _int, _int, C.POINTER(_int), _vp]),
     "fss_levels_for": (_int, [C.c_int64, _dbl]),
+    "fss_set_near_tile": (_int, [_vp, _int]),
     "fss_tree_export": (_int, [_vp] + [_vp] * 8),
     "fss_velocity_direct": (_int, [_vp, C.POINTER(Particles), C.POINTER(Velocity)]),
     "fss_velocity_tree": (_int, [_vp, C.POINTER(Particles), C.POINTER(Velocity)]),
@@ -304,6 +307,9 @@ class Context:
         self.n = x.numel()
         self.cfg.tree_levels = best.value
         return best.value, {lmin + k: float(v) for k, v in enumerate(ms)}
+
+    def set_near_tile(self, targets_per_lane):
+        _check(lib().fss_set_near_tile(self.h, int(targets_per_lane)))
 
     def stats(self):
         s = TreeStats()

@@ -103,6 +103,12 @@ typedef struct {
     int64_t pairs_far;         /* kernel evaluations of the far field (proxy work) */
     double mac_margin;         /* min |sep - theta| over every MAC with d > 0 */
     int n_generations;         /* traversal generations (FIFO depth) */
+    /* evaluation plan */
+    int64_t n_records;         /* particle source records (main + exception) */
+    int64_t n_near_items;      /* work items of the near kernel (<= 64 targets each) */
+    int64_t n_proxy_items;     /* work items of the proxy kernel */
+    int64_t n_interpolants;    /* nodes with proxy-target work */
+    int64_t n_charge_nodes;    /* nodes whose proxies carry modified charges (BLTC) */
 } fss_tree_stats;
 
 /* ---- context ---- */
@@ -148,6 +154,10 @@ fss_status fss_autotune_levels(fss_ctx *ctx, const fss_particles *p, int lmin, i
 /* Depth whose leaves hold `leaf_occupancy` particles on average for n
  * particles: round(1 + log4(n / (20 occupancy))), clamped to [1, 13]. */
 int fss_levels_for(int64_t n, double leaf_occupancy);
+/* Tile shape of the near-field kernel: targets held per lane (2 or 4; 0 =
+ * default: 4 in FP64, 2 in mixed precision).  A tuning knob; results agree
+ * to rounding. */
+fss_status fss_set_near_tile(fss_ctx *ctx, int targets_per_lane);
 
 /* Copies integer tree data to HOST buffers for bit-exact checks (any pointer
  * may be NULL to skip it):
