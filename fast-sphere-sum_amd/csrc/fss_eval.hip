@@ -517,8 +517,90 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
 // ------------------------------------------------------------------
 // proxy values (class B): p proxies of a node against one chunk of sources
 // ------------------------------------------------------------------
+// One lane per (item, proxy): a wave holds 64 / p items and every lane walks
+// its item's source ranges itself (the p lanes of an item read the same
+// source records, a broadcast).  No cross-lane reduction.  FP32 proxy source
+// ranges (cc) move to the target frame with one shift per range; particle
+// ranges per record.
 template <typename Real, int D>
 __global__ void __launch_bounds__(64) k_proxy(const ProxyItem *__restrict__ items,
+                                              const int32_t *__restrict__ list, int64_t nlist,
+                                              const Range *__restrict__ ranges,
+                                              const typename Rec<Real>::T4 *__restrict__ src,
+                                              const int32_t *__restrict__ recnode,
+                                              const float *__restrict__ aoff, int64_t p0,
+                                              const double *__restrict__ centres,
+                                              const double4 *__restrict__ proxy_xyz, double *phi)
+{
+    using T4 = typename Rec<Real>::T4;
+    constexpr int PP = npts<D>();
+    constexpr int IPW = 64 / PP;
+    const int lane = threadIdx.x;
+    const int il = lane / PP, k = lane - il * PP;
+    const int64_t li = (int64_t)blockIdx.x * IPW + il;
+    if (il >= IPW || li >= nlist) return;
+    const int64_t item = list[li];
+    const ProxyItem it = items[item];
+    const int64_t pk = p0 + (int64_t)it.node * PP + k;
+    const T4 yk = src[pk];
+    const Real at = aoff ? (Real)aoff[pk - p0] : (Real)0;
+    const double *cp = centres + 4 * (int64_t)it.node;
+    const double ct[3] = {cp[0], cp[1], cp[2]};
+    Acc<Real, 1> A;
+    A.zero();
+    Real *pq = A.part(0);
+    for (int r = it.b_beg; r < it.b_end; r++) {
+        const Range rg = ranges[r];
+        const int cnt = rg.count & ~FSS_SELF_BIT;
+        if (rg.start >= p0) {
+            // the p proxies of one source node (count == p): loads issued in
+            // batches of up to 8, one frame shift for the range
+            T4 sh;
+            sh.x = sh.y = sh.z = sh.w = 0;
+            sh = to_frame(sh, rg.node, centres, ct);
+            constexpr int B = PP < 8 ? PP : 8;
+#pragma unroll
+            for (int j0 = 0; j0 < PP; j0 += B) {
+                T4 s[B];
+#pragma unroll
+                for (int j = 0; j < B; j++)
+                    if (j0 + j < PP) s[j] = src[rg.start + j0 + j];
+#pragma unroll
+                for (int j = 0; j < B; j++) {
+                    if (j0 + j >= PP) continue;
+                    if constexpr (sizeof(Real) == 4) {
+                        s[j].x += sh.x;
+                        s[j].y += sh.y;
+                        s[j].z += sh.z;
+                    }
+                    pair_far(yk, s[j], at, pq[0], pq[1], pq[2]);
+                }
+            }
+            A.flush();
+        } else {
+            for (int j = 0; j < cnt; j++) {
+                const int64_t js = (int64_t)rg.start + j;
+                pair_far(yk, to_frame(src[js], recnode[js], centres, ct), at, pq[0], pq[1], pq[2]);
+                if ((j & 15) == 15) A.flush(); // FP32 partial sums of <= 16 terms
+            }
+            A.flush();
+        }
+    }
+    // y x x_j = y x (x_j - y): absolute FP64 proxy position
+    const double4 ya = proxy_xyz[(int64_t)it.node * PP + k];
+    const double a0 = A.total(0, 0), a1 = A.total(0, 1), a2 = A.total(0, 2);
+    double *o = phi + item * 3 * PP;
+    o[k] = ya.y * a2 - ya.z * a1;
+    o[PP + k] = ya.z * a0 - ya.x * a2;
+    o[2 * PP + k] = ya.x * a1 - ya.y * a0;
+}
+
+// Items with long particle source ranges (reference dispatch: cp and cc -> cp
+// take every particle of the source node): sources staged in LDS in chunks,
+// lanes split into proxy groups x source phases, reduction over phases.
+template <typename Real, int D>
+__global__ void __launch_bounds__(64) k_proxy_staged(const ProxyItem *__restrict__ items,
+                                                     const int32_t *__restrict__ list,
                                               const Range *__restrict__ ranges,
                                               const typename Rec<Real>::T4 *__restrict__ src,
                                               const int32_t *__restrict__ recnode,
@@ -531,7 +613,8 @@ __global__ void __launch_bounds__(64) k_proxy(const ProxyItem *__restrict__ item
     constexpr int R = PP > 6 ? 2 : 1;
     __shared__ T4 lds[NCH];
     __shared__ double red[64][3 * R];
-    const ProxyItem it = items[blockIdx.x];
+    const int32_t item = list[blockIdx.x];
+    const ProxyItem it = items[item];
     const int lane = threadIdx.x;
     const int T = (PP + R - 1) / R;
     const int P = 64 / T;
@@ -572,7 +655,7 @@ __global__ void __launch_bounds__(64) k_proxy(const ProxyItem *__restrict__ item
         for (int c = 0; c < 3; c++) red[lane][3 * q + c] = A.total(q, c);
     __syncthreads();
     if (ph == 0) {
-        double *o = phi + (int64_t)blockIdx.x * 3 * PP;
+        double *o = phi + (int64_t)item * 3 * PP;
 #pragma unroll
         for (int q = 0; q < R; q++) {
             int k = tl + q * T;
@@ -783,10 +866,18 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
                                    src);
         }
         if (T.n_proxy > 0)
-            hipLaunchKernelGGL((k_proxy<double, D>), dim3((unsigned)T.n_proxy), dim3(64), 0, s,
-                               T.proxy_items.as<ProxyItem>(), T.ranges_b.as<Range>(), src,
-                               T.recnode.as<int32_t>(), (const float *)nullptr, p0, T.tri_info.as<double>(),
-                               T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
+        {
+            if (T.n_plane > 0)
+                hipLaunchKernelGGL((k_proxy<double, D>), dim3((unsigned)div_up(T.n_plane, 64 / p)), dim3(64), 0, s,
+                                   T.proxy_items.as<ProxyItem>(), T.plist.as<int32_t>(), T.n_plane,
+                                   T.ranges_b.as<Range>(), src, T.recnode.as<int32_t>(), (const float *)nullptr, p0,
+                                   T.tri_info.as<double>(), T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
+            if (T.n_proxy > T.n_plane)
+                hipLaunchKernelGGL((k_proxy_staged<double, D>), dim3((unsigned)(T.n_proxy - T.n_plane)), dim3(64), 0,
+                                   s, T.proxy_items.as<ProxyItem>(), T.plist.as<int32_t>() + T.n_plane,
+                                   T.ranges_b.as<Range>(), src, T.recnode.as<int32_t>(), (const float *)nullptr, p0,
+                                   T.tri_info.as<double>(), T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
+        }
         if (ev) FSS_HIP(hipEventRecord(ev[1], s));
         if (T.n_near > 0) {
             auto kn = ctx->near_r == 2 ? k_near<double, 2> : k_near<double, 4>; // default 4
@@ -814,10 +905,18 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
                                    src);
         }
         if (T.n_proxy > 0)
-            hipLaunchKernelGGL((k_proxy<float, D>), dim3((unsigned)T.n_proxy), dim3(64), 0, s,
-                               T.proxy_items.as<ProxyItem>(), T.ranges_b.as<Range>(), src,
-                               T.recnode.as<int32_t>(), (const float *)aoff, p0, T.tri_info.as<double>(),
-                               T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
+        {
+            if (T.n_plane > 0)
+                hipLaunchKernelGGL((k_proxy<float, D>), dim3((unsigned)div_up(T.n_plane, 64 / p)), dim3(64), 0, s,
+                                   T.proxy_items.as<ProxyItem>(), T.plist.as<int32_t>(), T.n_plane,
+                                   T.ranges_b.as<Range>(), src, T.recnode.as<int32_t>(), (const float *)aoff, p0,
+                                   T.tri_info.as<double>(), T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
+            if (T.n_proxy > T.n_plane)
+                hipLaunchKernelGGL((k_proxy_staged<float, D>), dim3((unsigned)(T.n_proxy - T.n_plane)), dim3(64), 0,
+                                   s, T.proxy_items.as<ProxyItem>(), T.plist.as<int32_t>() + T.n_plane,
+                                   T.ranges_b.as<Range>(), src, T.recnode.as<int32_t>(), (const float *)aoff, p0,
+                                   T.tri_info.as<double>(), T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
+        }
         if (ev) FSS_HIP(hipEventRecord(ev[1], s));
         if (T.n_near > 0) {
             auto kn = ctx->near_r == 4 ? k_near<float, 4> : k_near<float, 2>; // default 2

@@ -682,7 +682,7 @@ void tree_release(Tree &t)
                       &t.recnode, &t.nrange,
                       &t.near_items, &t.proxy_items, &t.ranges_a,
                       &t.ranges_b, &t.charge_nodes, &t.proxy_xyz, &t.up_items, &t.up_multi, &t.tinv,
-                      &t.rinfo, &t.oslot};
+                      &t.rinfo, &t.oslot, &t.plist};
     for (DevBuf *b : bufs) b->release();
     t.built = false;
 }
@@ -1022,16 +1022,22 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
         if (T.n_rb > 0) FSS_TRY(d2h(rb.data(), T.ranges_b.p, sizeof(Range) * T.n_rb, s));
         std::vector<ProxyItem> parts;
         std::vector<ProxySlot> slots(T.n_slot);
+        std::vector<int32_t> lane, staged;
         for (int64_t q = 0; q < T.n_slot; q++) {
             const ProxyItem &it = one[q];
             slots[q].node = it.node;
             slots[q].part_beg = (int32_t)parts.size();
             slots[q].pad = 0;
+            // a node with a long particle range goes to the LDS-staged kernel
+            bool longr = false;
+            for (int32_t r = it.b_beg; r < it.b_end; r++)
+                longr = longr || (rb[r].start < T.p0 && (rb[r].count & ~FSS_SELF_BIT) > FSS_LANE_MAX);
             int32_t b = it.b_beg;
             int64_t acc = 0;
             for (int32_t r = it.b_beg; r < it.b_end; r++) {
                 acc += rb[r].count & ~FSS_SELF_BIT;
                 if (acc >= FSS_PROXY_CHUNK || r + 1 == it.b_end) {
+                    (longr ? staged : lane).push_back((int32_t)parts.size());
                     parts.push_back(ProxyItem{it.node, b, r + 1, (int32_t)q});
                     b = r + 1;
                     acc = 0;
@@ -1040,6 +1046,11 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
             slots[q].part_end = (int32_t)parts.size();
         }
         T.n_proxy = (int64_t)parts.size();
+        T.n_plane = (int64_t)lane.size();
+        lane.insert(lane.end(), staged.begin(), staged.end());
+        FSS_TRY(T.plist.ensure(sizeof(int32_t) * std::max<int64_t>(1, T.n_proxy)));
+        if (T.n_proxy > 0)
+            FSS_HIP(hipMemcpyAsync(T.plist.p, lane.data(), sizeof(int32_t) * T.n_proxy, hipMemcpyHostToDevice, s));
         FSS_TRY(T.proxy_items.ensure(sizeof(ProxyItem) * std::max<int64_t>(1, T.n_proxy)));
         FSS_TRY(T.proxy_slots.ensure(sizeof(ProxySlot) * std::max<int64_t>(1, T.n_slot)));
         if (T.n_proxy > 0)
