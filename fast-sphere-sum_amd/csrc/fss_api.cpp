@@ -444,6 +444,23 @@ fss_status fss_rk4_run_graph(fss_ctx *ctx, double *x, double *y, double *z, doub
     })
 }
 
+fss_status fss_remesh(fss_ctx *ctx, const double *ax, const double *ay, const double *az, const double *avort,
+                      const double *atracer, const int32_t *tris, int64_t ntri, const int32_t *vt_off,
+                      const int32_t *vt_idx, const double *x, const double *y, const double *z, double *vort,
+                      double *tracer, int32_t *tri_out, int64_t n, int64_t *n_fallback, int64_t *n_missing)
+{
+    FSS_GUARD({
+        if (!ctx || !ax || !ay || !az || !avort || !tris || !vt_off || !vt_idx || !x || !y || !z || !vort || n < 1 ||
+            ntri < 1)
+            return set_error("null argument"), FSS_ERR_INVALID;
+        if (!atracer != !tracer) return set_error("remesh: atracer and tracer go together"), FSS_ERR_INVALID;
+        if (ntri > INT32_MAX || n > INT32_MAX) return set_error("remesh: sizes exceed 32-bit indices"), FSS_ERR_INVALID;
+        FSS_HIP(hipSetDevice(ctx->cfg.device));
+        return remesh(ctx, ax, ay, az, avort, atracer, tris, ntri, vt_off, vt_idx, x, y, z, vort, tracer, tri_out,
+                      n, ctx->cfg.omega, n_fallback, n_missing);
+    })
+}
+
 fss_status fss_project_points(fss_ctx *ctx, double *x, double *y, double *z, double *vort, int64_t n)
 {
     if (!ctx || !x || !y || !z || !vort || n < 1) return set_error("null argument"), FSS_ERR_INVALID;

@@ -203,6 +203,23 @@ fss_status fss_grid_voronoi_areas(int64_t n, const double *x, const double *y, c
     return FSS_OK;
 }
 
+fss_status fss_grid_vertex_triangles(int64_t n, const int32_t *tris, int64_t ntri, int32_t *vt_off,
+                                     int32_t *vt_idx)
+{
+    if (n < 1 || ntri < 1 || !tris || !vt_off || !vt_idx) return set_error("bad arguments"), FSS_ERR_INVALID;
+    std::vector<int32_t> cnt(n + 1, 0);
+    for (int64_t k = 0; k < 3 * ntri; k++) {
+        if (tris[k] < 0 || tris[k] >= n) return set_error("triangle index out of range"), FSS_ERR_INVALID;
+        cnt[tris[k] + 1]++;
+    }
+    vt_off[0] = 0;
+    for (int64_t i = 0; i < n; i++) vt_off[i + 1] = vt_off[i] + cnt[i + 1];
+    std::vector<int32_t> fill(vt_off, vt_off + n);
+    for (int64_t t = 0; t < ntri; t++) // ascending triangle index per vertex
+        for (int m = 0; m < 3; m++) vt_idx[fill[tris[3 * t + m]]++] = (int32_t)t;
+    return FSS_OK;
+}
+
 // helpers.h:94-115 and 344-360
 fss_status fss_grid_third_areas(int64_t n, const double *x, const double *y, const double *z,
                                 const int32_t *tris, int64_t ntri, double *area)

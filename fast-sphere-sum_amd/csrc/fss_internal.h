@@ -303,6 +303,12 @@ fss_status rk4_run_graph(fss_ctx *ctx, double *x, double *y, double *z, double *
                          const double *area, int64_t n, double dt, int use_tree, int steps);
 fss_status project_points(fss_ctx *ctx, double *x, double *y, double *z, double *vort, int64_t n);
 
+// ---- remesh (fss_remesh.hip) ----
+fss_status remesh(fss_ctx *ctx, const double *ax, const double *ay, const double *az, const double *avort,
+                  const double *atr, const int32_t *tris, int64_t ntri, const int32_t *vt_off,
+                  const int32_t *vt_idx, const double *x, const double *y, const double *z, double *vort,
+                  double *tr, int32_t *tri_out, int64_t n, double omega, int64_t *n_fallback, int64_t *n_missing);
+
 // ---- leaf-size autotuner (fss_tune.cpp) ----
 int levels_for(int64_t n, double occupancy);
 fss_status autotune_levels(fss_ctx *ctx, const fss_particles *pp, int lmin, int lmax, int reps, int *best,

@@ -92,6 +92,9 @@ SIGNATURES = {
     "fss_grid_icosahedral": (_int, [_int, _vp, _vp, _vp, _vp]),
     "fss_grid_voronoi_areas": (_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp]),
     "fss_grid_third_areas": (_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "fss_grid_vertex_triangles": (_int, [_i64, _vp, _i64, _vp, _vp]),
+    "fss_remesh": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                          _i64, C.POINTER(_i64), C.POINTER(_i64)]),
     "fss_grid_random": (_int, [_i64, _u64, _vp, _vp, _vp]),
     "fss_field_rossby_haurwitz": (_int, [_i64, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp]),
     "fss_field_gauss_vortices": (_int, [_i64, _vp, _vp, _vp, _vp, _int, _dbl, _dbl, _dbl, _u64,
@@ -177,6 +180,15 @@ def third_areas(x, y, z, tris):
     _check(lib().fss_grid_third_areas(len(x), _np_ptr(x), _np_ptr(y), _np_ptr(z), _np_ptr(tris),
                                       tris.shape[0], _np_ptr(a)))
     return a
+
+
+def vertex_triangles(n, tris):
+    """CSR (vt_off[n+1], vt_idx) of the triangles adjacent to every vertex."""
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    off = np.zeros(n + 1, dtype=np.int32)
+    idx = np.zeros(tris.size, dtype=np.int32)
+    _check(lib().fss_grid_vertex_triangles(n, _np_ptr(tris), tris.shape[0], _np_ptr(off), _np_ptr(idx)))
+    return off, idx
 
 
 def grid_random(n, seed):
@@ -365,6 +377,23 @@ class Context:
     def rk4_run_graph(self, x, y, z, vort, area, dt, steps, use_tree=False):
         _check(lib().fss_rk4_run_graph(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), _dev_ptr(vort),
                                        _dev_ptr(area), x.numel(), dt, int(use_tree), steps))
+
+    def remesh(self, ax, ay, az, avort, tris, vt_off, vt_idx, x, y, z, atracer=None, tri_out=False):
+        """Remesh the advected state (ax, ay, az, avort) onto the grid points
+        (x, y, z): returns (vort, tracer or None, triangles or None,
+        n_fallback).  tris / vt_off / vt_idx are device int32 tensors."""
+        import torch
+        n = x.numel()
+        vort = torch.empty(n, dtype=torch.float64, device=x.device)
+        tr = torch.empty_like(vort) if atracer is not None else None
+        to = torch.empty(n, dtype=torch.int32, device=x.device) if tri_out else None
+        nf, nm = _i64(), _i64()
+        _check(lib().fss_remesh(self.h, _dev_ptr(ax), _dev_ptr(ay), _dev_ptr(az), _dev_ptr(avort),
+                                _dev_ptr(atracer) if atracer is not None else None, _dev_ptr(tris),
+                                tris.numel() // 3, _dev_ptr(vt_off), _dev_ptr(vt_idx), _dev_ptr(x), _dev_ptr(y),
+                                _dev_ptr(z), _dev_ptr(vort), _dev_ptr(tr) if tr is not None else None,
+                                _dev_ptr(to) if to is not None else None, n, C.byref(nf), C.byref(nm)))
+        return vort, tr, to, nf.value
 
     def project_points(self, x, y, z, vort):
         _check(lib().fss_project_points(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), _dev_ptr(vort),

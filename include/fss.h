@@ -194,6 +194,22 @@ fss_status fss_rk4_step(fss_ctx *ctx, double *x, double *y, double *z, double *v
 /* Same, replayed `steps` times from a hipGraph captured on the first call. */
 fss_status fss_rk4_run_graph(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
                              const double *area, int64_t n, double dt, int use_tree, int steps);
+/* ---- remesh (regrid_points2 remesh.h:109-178 with tri_interp
+ * helpers.h:251-265, called after each RK4 step by new_dynamics.cpp:213-217) ----
+ * For every grid point (x, y, z)[i] (device, unchanged): the triangle of the
+ * advected mesh (positions ax, ay, az; triangles tris[3 * ntri]) containing it
+ * -- the triangles adjacent to vertex i (vt_off / vt_idx, device CSR) first, in
+ * that order, then all triangles in index order, first hit wins -- and the
+ * absolute vorticity avort + 2 Omega az interpolated with normalised
+ * barycentric coordinates: vort[i] = interpolant - 2 Omega z[i].  atracer /
+ * tracer (optional, may be NULL) are interpolated directly.  tri_out (device,
+ * optional) receives the triangle of every target.  n_fallback / n_missing
+ * (host, optional): targets that needed the full search / lie in no
+ * triangle (FSS_ERR_NUMERIC then; the reference throws). */
+fss_status fss_remesh(fss_ctx *ctx, const double *ax, const double *ay, const double *az, const double *avort,
+                      const double *atracer, const int32_t *tris, int64_t ntri, const int32_t *vt_off,
+                      const int32_t *vt_idx, const double *x, const double *y, const double *z, double *vort,
+                      double *tracer, int32_t *tri_out, int64_t n, int64_t *n_fallback, int64_t *n_missing);
 /* project_points (helpers.h:362-372) on device state. */
 fss_status fss_project_points(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
                               int64_t n);
@@ -219,6 +235,11 @@ fss_status fss_grid_voronoi_areas(int64_t n, const double *x, const double *y, c
 /* area_init rule (helpers.h:344-360): a third of each adjacent triangle. */
 fss_status fss_grid_third_areas(int64_t n, const double *x, const double *y, const double *z,
                                 const int32_t *tris, int64_t ntri, double *area);
+/* Triangles adjacent to every vertex as CSR (the vert_tris lists read by
+ * read_points helpers.h:299-342): vt_off[n + 1], vt_idx[3 * ntri], ascending
+ * triangle index per vertex. */
+fss_status fss_grid_vertex_triangles(int64_t n, const int32_t *tris, int64_t ntri, int32_t *vt_off,
+                                     int32_t *vt_idx);
 /* Uniform random points on the sphere (normalised Gaussian triples). */
 fss_status fss_grid_random(int64_t n, uint64_t seed, double *x, double *y, double *z);
 /* Rossby-Haurwitz vorticity 2 w0 sin(lat) - 30 K sin(lat) cos^4(lat) cos(4 lon)

@@ -1189,3 +1189,68 @@ ORC_API void orc_rk4_step(void *tree, int64_t n, double *x, double *y, double *z
     for (int s = 0; s < 4; s++) free(k[s]);
     free(st);
 }
+
+/* ------------------------------------------------------------------ */
+/* remesh: regrid_points2 remesh.h:109-178, tri_interp helpers.h:251-265 */
+/* ------------------------------------------------------------------ */
+/* check_in_tri helpers.h:159-163 */
+static int check_in_tri(const double *p1, const double *p2, const double *p3, const double *p, double *c)
+{
+    barycoords(p1, p2, p3, p, c);
+    return (c[0] >= 0) && (c[1] >= 0) && (c[2] >= 0);
+}
+
+/* tri_interp helpers.h:251-265: normalised barycentric coordinates
+ * (norm_barycoords :152-157, scaled by the reciprocal of their sum), absolute
+ * vorticity interpolated linearly, relative vorticity recovered at the target;
+ * the tracer (if any) interpolated directly */
+static void tri_interp(const int32_t *tv, const double *ax, const double *ay, const double *az,
+                       const double *avort, const double *atr, const double *t, const double *c,
+                       double Omega, double *vort_out, double *tr_out)
+{
+    double r = 1.0 / (c[0] + c[1] + c[2]);
+    double b[3] = {c[0] * r, c[1] * r, c[2] * r};
+    double absv[3];
+    for (int m = 0; m < 3; m++) absv[m] = avort[tv[m]] + 2 * Omega * az[tv[m]];
+    double a = b[0] * absv[0] + b[1] * absv[1] + b[2] * absv[2];
+    *vort_out = a - 2 * Omega * t[2];
+    if (atr && tr_out) *tr_out = b[0] * atr[tv[0]] + b[1] * atr[tv[1]] + b[2] * atr[tv[2]];
+    (void)ax;
+    (void)ay;
+}
+
+/* For every target i (grid point): the triangles adjacent to vertex i first
+ * (vt_idx[vt_off[i] .. vt_off[i+1]), in that order), then all triangles in
+ * index order; the first containing the target wins.  tri_out[i] = that
+ * triangle or -1 (the reference throws).  Returns the number of targets that
+ * needed the full search. */
+ORC_API int64_t orc_remesh(int64_t n, const double *ax, const double *ay, const double *az,
+                           const double *avort, const double *atr, const int32_t *tris, int64_t ntri,
+                           const int32_t *vt_off, const int32_t *vt_idx, const double *x, const double *y,
+                           const double *z, double Omega, double *vort, double *tr, int32_t *tri_out)
+{
+    int64_t failure = 0;
+    for (int64_t i = 0; i < n; i++) {
+        double t[3] = {x[i], y[i], z[i]}, c[3];
+        int32_t hit = -1;
+        for (int32_t j = vt_off[i]; j < vt_off[i + 1] && hit < 0; j++) {
+            const int32_t *tv = tris + 3 * (int64_t)vt_idx[j];
+            double v1[3] = {ax[tv[0]], ay[tv[0]], az[tv[0]]}, v2[3] = {ax[tv[1]], ay[tv[1]], az[tv[1]]},
+                   v3[3] = {ax[tv[2]], ay[tv[2]], az[tv[2]]};
+            if (check_in_tri(v1, v2, v3, t, c)) hit = vt_idx[j];
+        }
+        if (hit < 0) {
+            failure++;
+            for (int64_t j = 0; j < ntri && hit < 0; j++) {
+                const int32_t *tv = tris + 3 * j;
+                double v1[3] = {ax[tv[0]], ay[tv[0]], az[tv[0]]}, v2[3] = {ax[tv[1]], ay[tv[1]], az[tv[1]]},
+                       v3[3] = {ax[tv[2]], ay[tv[2]], az[tv[2]]};
+                if (check_in_tri(v1, v2, v3, t, c)) hit = (int32_t)j;
+            }
+        }
+        tri_out[i] = hit;
+        if (hit >= 0)
+            tri_interp(tris + 3 * (int64_t)hit, ax, ay, az, avort, atr, t, c, Omega, &vort[i], tr ? &tr[i] : NULL);
+    }
+    return failure;
+}

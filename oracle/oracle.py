@@ -86,6 +86,9 @@ def lib():
     L.orc_project_points.argtypes = [i64, _dp, _dp, _dp, _dp, f64]
     L.orc_area_init.restype = None
     L.orc_area_init.argtypes = [i64, _dp, _dp, _dp, _ip, i64, _dp]
+    L.orc_remesh.restype = i64
+    L.orc_remesh.argtypes = [i64, _dp, _dp, _dp, _dp, vp, _ip, i64, _ip, _ip, _dp, _dp, _dp, f64, _dp, vp,
+                             _ip]
     L.orc_rk4_step.restype = None
     L.orc_rk4_step.argtypes = [vp, i64, _dp, _dp, _dp, _dp, _dp, f64, f64, i32]
     _LIB = L
@@ -284,3 +287,32 @@ def rk4_step(x, y, z, vort, area, dt, omega=2 * np.pi, tree=None, nthreads=None)
     lib().orc_rk4_step(tree.h if tree is not None else None, a[0].shape[0], *a, _c(area), omega, dt,
                        nthreads)
     return tuple(a)
+
+
+def vertex_triangles(n, tris):
+    """Triangles adjacent to every vertex, ascending triangle index (the
+    vert_tris lists read by read_points helpers.h:299-342), as CSR."""
+    tris = np.asarray(tris, dtype=np.int64).reshape(-1, 3)
+    cnt = np.bincount(tris.ravel(), minlength=n)
+    off = np.zeros(n + 1, dtype=np.int32)
+    off[1:] = np.cumsum(cnt)
+    order = np.argsort(tris.ravel(), kind="stable")
+    idx = (order // 3).astype(np.int32)
+    return off, idx
+
+
+def remesh(ax, ay, az, avort, tris, vt_off, vt_idx, x, y, z, omega=2 * np.pi, atracer=None):
+    """regrid_points2 remesh.h:109-178 with tri_interp helpers.h:251-265:
+    returns (vort at the targets, tracer or None, triangle per target,
+    number of targets that needed the full search)."""
+    n = len(x)
+    tris = _c(tris, np.int32).ravel()
+    vort = np.zeros(n)
+    tr = np.zeros(n) if atracer is not None else None
+    tri_out = np.zeros(n, dtype=np.int32)
+    atr = _c(atracer) if atracer is not None else None
+    nf = lib().orc_remesh(n, _c(ax), _c(ay), _c(az), _c(avort),
+                          atr.ctypes.data_as(C.c_void_p) if atr is not None else None, tris,
+                          tris.shape[0] // 3, _c(vt_off, np.int32), _c(vt_idx, np.int32), _c(x), _c(y), _c(z),
+                          omega, vort, tr.ctypes.data_as(C.c_void_p) if tr is not None else None, tri_out)
+    return vort, tr, tri_out, int(nf)
