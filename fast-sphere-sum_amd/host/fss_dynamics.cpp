@@ -3,15 +3,16 @@
 //
 //   fss_dynamics [-f] [-c level] [-l tree_levels] [-h theta] [-t days] [-x]
 //                [-i rh4|gv] [-g] [-o dir]
-// Flags as run_config.h:9-51 (-m, -a, -r, -b are accepted and ignored: MPI,
-// AMR and remeshing are not part of this build), plus
+// Flags as run_config.h:9-51 (-m, -a, -b are accepted and ignored: MPI and AMR
+// are not part of this build; -r remeshes onto the grid after every step,
+// new_dynamics.cpp:213-217), plus
 //   -g  run the RK4 step on the device (fused update + projection kernels
 //       replayed from a hipGraph) instead of one rhs_func round trip per stage
 //   -o  output directory (default .)
 // The input files of the reference (read_points, helpers.h:299-342) are not
 // available, so the particles are the icosahedral vertices of level -c, with
 // area_init areas (helpers.h:344-360) and the reference's initial conditions
-// (vorticity_functions.h:13-54).  Output: output_<N>_<ic>_{fast_<L>|direct}_<steps>.csv,
+// (vorticity_functions.h:13-54).  Output: output_<N>_<ic>_{fast_<L>|direct}[_remesh]_<steps>.csv,
 // rows x, y, z, vorticity, tracer, area for the initial state and every step
 // (new_dynamics.cpp:124-140, 230-236).
 #include <unistd.h>
@@ -112,6 +113,7 @@ int main(int argc, char **argv)
 
         std::string out_config = std::to_string(n) + "_" + config1.init_cond + "_";
         out_config += config1.use_fast ? "fast_" + std::to_string(config1.tree_levels) : "direct";
+        if (config1.use_remesh) out_config += "_remesh";
         out_config += "_" + std::to_string(time_steps);
         std::ofstream w1(outdir + "/output_" + out_config + ".csv", std::ofstream::trunc);
         std::ofstream w2(outdir + "/point_counts_" + out_config + ".csv", std::ofstream::trunc);
@@ -125,14 +127,64 @@ int main(int argc, char **argv)
         };
         write_state();
 
+        // remesh topology: the grid triangles and the triangles adjacent to
+        // every vertex (read from {N}vert_tris.csv by read_points in the reference)
+        std::vector<int32_t> vt_off(n + 1), vt_idx(3 * (size_t)config1.tri_count);
+        std::vector<std::vector<int>> dyn_tris, vert_tris;
+        if (config1.use_remesh) {
+            check(fss_grid_vertex_triangles(n, tris.data(), config1.tri_count, vt_off.data(), vt_idx.data()),
+                  "fss_grid_vertex_triangles");
+            dyn_tris.assign(config1.tri_count, std::vector<int>(3));
+            for (int t = 0; t < config1.tri_count; t++)
+                for (int m = 0; m < 3; m++) dyn_tris[t][m] = tris[3 * t + m];
+            vert_tris.assign(n, std::vector<int>());
+            for (int i = 0; i < n; i++) vert_tris[i].assign(vt_idx.begin() + vt_off[i], vt_idx.begin() + vt_off[i + 1]);
+        }
+
         auto t0 = std::chrono::steady_clock::now();
         if (device_rk4) {
             device_state &s = detail::ensure_device(config1);
             detail::upload(s, curr_state, &area);
+            // remesh buffers: grid positions, tracer, topology, outputs
+            double *g[6] = {};
+            int32_t *gi[2] = {};
+            if (config1.use_remesh) {
+                for (double *&q : g) hcheck(hipMalloc(&q, sizeof(double) * n), "hipMalloc");
+                hcheck(hipMalloc(&gi[0], sizeof(int32_t) * tris.size()), "hipMalloc");
+                hcheck(hipMalloc(&gi[1], sizeof(int32_t) * (vt_off.size() + vt_idx.size())), "hipMalloc");
+                hcheck(hipMemcpy(g[0], x.data(), sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+                hcheck(hipMemcpy(g[1], y.data(), sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+                hcheck(hipMemcpy(g[2], z.data(), sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+                std::vector<double> tr(n);
+                for (int i = 0; i < n; i++) tr[i] = curr_state[5 * i + 4];
+                hcheck(hipMemcpy(g[3], tr.data(), sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+                hcheck(hipMemcpy(gi[0], tris.data(), sizeof(int32_t) * tris.size(), hipMemcpyHostToDevice), "hipMemcpy");
+                hcheck(hipMemcpy(gi[1], vt_off.data(), sizeof(int32_t) * vt_off.size(), hipMemcpyHostToDevice),
+                       "hipMemcpy");
+                hcheck(hipMemcpy(gi[1] + vt_off.size(), vt_idx.data(), sizeof(int32_t) * vt_idx.size(),
+                                 hipMemcpyHostToDevice),
+                       "hipMemcpy");
+            }
             for (int t = 0; t < time_steps; t++) {
                 check(fss_rk4_run_graph(s.ctx, s.d[0], s.d[1], s.d[2], s.d[3], s.d[4], n, delta_t,
                                         config1.use_fast ? 1 : 0, 1),
                       "fss_rk4_run_graph");
+                if (config1.use_remesh) { // new_dynamics.cpp:213-217, on the device
+                    int64_t nf = 0, nm = 0;
+                    check(fss_remesh(s.ctx, s.d[0], s.d[1], s.d[2], s.d[3], g[3], gi[0], config1.tri_count, gi[1],
+                                     gi[1] + vt_off.size(), g[0], g[1], g[2], g[4], g[5], nullptr, n, &nf, &nm),
+                          "fss_remesh");
+                    hcheck(hipMemcpy(s.d[3], g[4], sizeof(double) * n, hipMemcpyDeviceToDevice), "hipMemcpy");
+                    hcheck(hipMemcpy(g[3], g[5], sizeof(double) * n, hipMemcpyDeviceToDevice), "hipMemcpy");
+                    for (int c = 0; c < 3; c++)
+                        hcheck(hipMemcpy(s.d[c], g[c], sizeof(double) * n, hipMemcpyDeviceToDevice), "hipMemcpy");
+                    check(fss_project_points(s.ctx, s.d[0], s.d[1], s.d[2], s.d[3], n), "fss_project_points");
+                    check(fss_sync(s.ctx), "fss_sync");
+                    std::vector<double> tr(n);
+                    hcheck(hipMemcpy(tr.data(), g[3], sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy");
+                    for (int i = 0; i < n; i++) curr_state[5 * i + 4] = tr[i];
+                    std::cout << "success: " << n - nf << " failure: " << nf << " bad: " << nm << std::endl;
+                }
                 std::vector<double> h(n);
                 for (int c = 0; c < 4; c++) {
                     hcheck(hipMemcpy(h.data(), s.d[c], sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy");
@@ -141,6 +193,10 @@ int main(int argc, char **argv)
                 write_state();
                 std::cout << "t: " << t << " point_count " << n << std::endl;
             }
+            for (double *q : g)
+                if (q) (void)hipFree(q);
+            for (int32_t *q : gi)
+                if (q) (void)hipFree(q);
         } else {
             std::vector<double> c1(5 * (size_t)n), c2(c1), c3(c1), c4(c1), inter(c1);
             const double fac[3] = {delta_t / 2, delta_t / 2, delta_t};
@@ -159,7 +215,12 @@ int main(int argc, char **argv)
                     a += c3[i] * 2;
                     a += c4[i];
                     a *= delta_t / 6;
-                    curr_state[i] += a;
+                    if (config1.use_remesh) inter[i] = curr_state[i] + a; // advected state (c1234)
+                    else curr_state[i] += a;
+                }
+                if (config1.use_remesh) { // new_dynamics.cpp:213-217
+                    project_points(inter, n, omega);
+                    regrid_points2(inter, curr_state, dyn_tris, vert_tris, n, config1.tri_count, omega, 0, n, 0);
                 }
                 project_points(curr_state, n, omega);
                 write_state();

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <iostream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -245,6 +246,82 @@ inline void rhs_func(std::vector<double> &modify, std::vector<double> &curr_stat
         modify[5 * i + 2] = u[2 * n + i];
         modify[5 * i + 3] = -2 * omega * u[2 * n + i];
     }
+}
+
+// remesh.h:109-178 (regrid_points2, with tri_interp helpers.h:251-265): the
+// grid points in target_points (AoS stride 5) take the absolute vorticity and
+// tracer of the advected state curr_state interpolated on the advected mesh
+// (triangles, vert_tris); the search runs on the device (fss_remesh).  Only
+// the whole range is supported (lb = 0, ub = point_count), as the driver calls
+// it; ID is unused.  Throws the failing target's index like the reference when
+// a target lies in no triangle.
+inline void regrid_points2(std::vector<double> &curr_state, std::vector<double> &target_points,
+                           std::vector<std::vector<int>> &triangles, std::vector<std::vector<int>> &vert_tris,
+                           int point_count, int tri_count, double omega, int lb, int ub, int ID)
+{
+    (void)ID;
+    if (lb != 0 || ub != point_count) throw std::runtime_error("regrid_points2: only the full range");
+    const int64_t n = point_count;
+    std::vector<int32_t> tris(3 * (size_t)tri_count), off(n + 1, 0), idx;
+    for (int t = 0; t < tri_count; t++)
+        for (int m = 0; m < 3; m++) tris[3 * t + m] = triangles[t][m];
+    for (int64_t i = 0; i < n; i++) {
+        off[i + 1] = off[i] + (int32_t)vert_tris[i].size();
+        for (int t : vert_tris[i]) idx.push_back(t);
+    }
+    // device arrays: advected x y z vort tracer, grid x y z, out vort tracer, topology
+    double *d[10] = {};
+    int32_t *di[4] = {};
+    auto cleanup = [&]() {
+        for (double *q : d)
+            if (q) (void)hipFree(q);
+        for (int32_t *q : di)
+            if (q) (void)hipFree(q);
+    };
+    try {
+        for (double *&q : d) hcheck(hipMalloc(&q, sizeof(double) * n), "hipMalloc");
+        hcheck(hipMalloc(&di[0], sizeof(int32_t) * tris.size()), "hipMalloc");
+        hcheck(hipMalloc(&di[1], sizeof(int32_t) * off.size()), "hipMalloc");
+        hcheck(hipMalloc(&di[2], sizeof(int32_t) * std::max<size_t>(1, idx.size())), "hipMalloc");
+        hcheck(hipMalloc(&di[3], sizeof(int32_t) * n), "hipMalloc");
+        std::vector<double> h(n);
+        for (int c = 0; c < 5; c++) {
+            for (int64_t i = 0; i < n; i++) h[i] = curr_state[5 * i + c];
+            hcheck(hipMemcpy(d[c], h.data(), sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+        }
+        for (int c = 0; c < 3; c++) {
+            for (int64_t i = 0; i < n; i++) h[i] = target_points[5 * i + c];
+            hcheck(hipMemcpy(d[5 + c], h.data(), sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+        }
+        hcheck(hipMemcpy(di[0], tris.data(), sizeof(int32_t) * tris.size(), hipMemcpyHostToDevice), "hipMemcpy");
+        hcheck(hipMemcpy(di[1], off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice), "hipMemcpy");
+        if (!idx.empty())
+            hcheck(hipMemcpy(di[2], idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice), "hipMemcpy");
+        fss_config cfg;
+        check(fss_config_default(&cfg), "fss_config_default");
+        cfg.omega = omega;
+        fss_ctx *ctx = nullptr;
+        check(fss_create(&ctx, &cfg), "fss_create");
+        int64_t nf = 0, nm = 0;
+        fss_status st = fss_remesh(ctx, d[0], d[1], d[2], d[3], d[4], di[0], tri_count, di[1], di[2], d[5], d[6],
+                                   d[7], d[8], d[9], di[3], n, &nf, &nm);
+        fss_destroy(ctx);
+        std::vector<int32_t> tri(n);
+        hcheck(hipMemcpy(tri.data(), di[3], sizeof(int32_t) * n, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (st == FSS_ERR_NUMERIC)
+            for (int64_t i = 0; i < n; i++)
+                if (tri[i] < 0) throw(int) i;
+        check(st, "fss_remesh");
+        for (int c = 0; c < 2; c++) {
+            hcheck(hipMemcpy(h.data(), d[8 + c], sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy");
+            for (int64_t i = 0; i < n; i++) target_points[5 * i + 3 + c] = h[i];
+        }
+        std::cout << "success: " << n - nf << " failure: " << nf << " bad: " << nm << std::endl;
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
 }
 
 // helpers.h:362-372 (host, AoS stride 5)

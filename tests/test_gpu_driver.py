@@ -41,3 +41,27 @@ def test_driver_step_matches_oracle(orc, tmp_path, args):
     for c in range(4):
         assert rel_l2(s1[:, c], ref[c]) <= 1e-12, c
     np.testing.assert_array_equal(s1[:, 4], tracer)  # tracer is carried unchanged
+
+
+@pytest.mark.parametrize("device", [False, True])
+def test_driver_remesh_step_matches_oracle(orc, tmp_path, device):
+    """-r (new_dynamics.cpp:213-217): after the RK4 step the particles go back
+    to the grid and carry the remeshed vorticity and tracer; the oracle's RK4
+    followed by its remesh gives the same state."""
+    import oracle
+    import fss
+    args = ["-x", "-c", "3", "-r"] + (["-g"] if device else [])
+    data, name = _run(tmp_path, args)
+    n = 10 * 4 ** 3 + 2
+    assert "_remesh_" in name and data.shape == (2 * n, 6)
+    s0, s1 = data[:n], data[n:]
+    x, y, z, vort, tracer, area = s0.T
+    ax, ay, az, av = orc.rk4_step(x, y, z, vort, area, 0.01)
+    _, _, _, tris = fss.grid_icosahedral(3)
+    off, idx = oracle.vertex_triangles(n, tris)
+    want_v, want_t, tri, nf = oracle.remesh(ax, ay, az, av, tris, off, idx, x, y, z, atracer=tracer)
+    assert (tri >= 0).all()
+    want = orc.project_points(x, y, z, want_v)
+    np.testing.assert_allclose(s1[:, :3], s0[:, :3], atol=1e-15)  # back on the grid
+    assert rel_l2(s1[:, 3], want[3]) <= 1e-12
+    assert rel_l2(s1[:, 4], want_t) <= 1e-12
