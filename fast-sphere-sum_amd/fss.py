@@ -93,6 +93,8 @@ SIGNATURES = {
     "fss_grid_voronoi_areas": (_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp]),
     "fss_grid_third_areas": (_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp]),
     "fss_grid_vertex_triangles": (_int, [_i64, _vp, _i64, _vp, _vp]),
+    "fss_amr": (_int, [C.POINTER(_i64), C.POINTER(_i64), _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                       _vp, _vp, _int, _dbl, _dbl, C.POINTER(_i64)]),
     "fss_remesh": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                           _i64, C.POINTER(_i64), C.POINTER(_i64)]),
     "fss_grid_random": (_int, [_i64, _u64, _vp, _vp, _vp]),
@@ -189,6 +191,24 @@ def vertex_triangles(n, tris):
     idx = np.zeros(tris.size, dtype=np.int32)
     _check(lib().fss_grid_vertex_triangles(n, _np_ptr(tris), tris.shape[0], _np_ptr(off), _np_ptr(idx)))
     return off, idx
+
+
+def amr(state, area, tris, parent, vert_tris, vert_tri_count, n, ntri, max_count, tri_cap,
+        circulation_tol=0.0025, vorticity_tol=0.2):
+    """One refinement pass (amr.h:13-178) on host arrays, edited in place:
+    state = [x, y, z, vort, tracer] (capacity max_count each), area,
+    tris [tri_cap, 3] int32, parent [max_count, 2] int32, vert_tris
+    [max_count, stride] int32, vert_tri_count [max_count] int32.  Returns
+    (n, ntri, refined)."""
+    nn, nt, nr = _i64(n), _i64(ntri), _i64()
+    arrs = [np.ascontiguousarray(a) for a in state]
+    for a, b in zip(arrs, state):
+        assert a is b, "state arrays must be contiguous"
+    _check(lib().fss_amr(C.byref(nn), C.byref(nt), max_count, tri_cap, *[_np_ptr(a) for a in state],
+                         _np_ptr(area), _np_ptr(tris), _np_ptr(parent), _np_ptr(vert_tris),
+                         _np_ptr(vert_tri_count), vert_tris.shape[1], circulation_tol, vorticity_tol,
+                         C.byref(nr)))
+    return nn.value, nt.value, nr.value
 
 
 def grid_random(n, seed):

@@ -84,6 +84,11 @@ struct run_information {
     int eval_mode = FSS_EVAL_PARITY;
     int node_rule = FSS_NODES_REFERENCE;
     std::shared_ptr<device_state> dev;
+    // remeshing and AMR topology (struct_list.h: dynamics_tris, amr_vert_tris,
+    // amr_parent_verts)
+    std::vector<std::vector<int>> dynamics_tris;
+    std::vector<std::vector<int>> amr_vert_tris;
+    std::vector<std::vector<int>> amr_parent_verts;
 };
 
 inline void check(fss_status s, const char *what)
@@ -322,6 +327,56 @@ inline void regrid_points2(std::vector<double> &curr_state, std::vector<double> 
         throw;
     }
     cleanup();
+}
+
+// amr.h:13-178: one refinement pass on the AoS stride-5 state, the areas and
+// the topology in config1 (dynamics_tris, amr_vert_tris, amr_parent_verts
+// sized max_count); runs fss_amr on flat copies.
+inline void amr(std::vector<double> &curr_state, std::vector<double> &areas, run_information &config1)
+{
+    const int64_t cap = std::max<int64_t>(config1.max_count, config1.point_count);
+    int64_t n = config1.point_count, T = config1.tri_count;
+    if (n >= config1.max_count) return;
+    const int stride = 12;
+    const int64_t tcap = T + 3 * (int64_t)T;
+    std::vector<double> st[5];
+    for (int c = 0; c < 5; c++) {
+        st[c].assign(cap, 0.0);
+        for (int64_t i = 0; i < n; i++) st[c][i] = curr_state[5 * i + c];
+    }
+    std::vector<double> ar(cap, 0.0);
+    std::copy(areas.begin(), areas.begin() + n, ar.begin());
+    std::vector<int32_t> tris(3 * tcap), parent(2 * cap, 0), vt(stride * cap, 0), vtc(cap, 0);
+    for (int64_t t = 0; t < T; t++)
+        for (int m = 0; m < 3; m++) tris[3 * t + m] = config1.dynamics_tris[t][m];
+    for (int64_t i = 0; i < (int64_t)config1.amr_parent_verts.size() && i < cap; i++) {
+        parent[2 * i] = config1.amr_parent_verts[i][0];
+        parent[2 * i + 1] = config1.amr_parent_verts[i][1];
+    }
+    for (int64_t i = 0; i < n; i++) {
+        const auto &l = config1.amr_vert_tris[i];
+        if ((int)l.size() > stride) throw std::runtime_error("amr: vertex with more than 12 triangles");
+        vtc[i] = (int32_t)l.size();
+        for (size_t k = 0; k < l.size(); k++) vt[stride * i + k] = l[k];
+    }
+    int64_t refined = 0;
+    check(fss_amr(&n, &T, config1.max_count, tcap, st[0].data(), st[1].data(), st[2].data(), st[3].data(),
+                  st[4].data(), ar.data(), tris.data(), parent.data(), vt.data(), vtc.data(), stride, 0.0025, 0.2,
+                  &refined),
+          "fss_amr");
+    curr_state.assign(5 * n, 0.0);
+    for (int64_t i = 0; i < n; i++)
+        for (int c = 0; c < 5; c++) curr_state[5 * i + c] = st[c][i];
+    areas.assign(ar.begin(), ar.begin() + n);
+    config1.dynamics_tris.assign(T, std::vector<int>(3));
+    for (int64_t t = 0; t < T; t++)
+        for (int m = 0; m < 3; m++) config1.dynamics_tris[t][m] = tris[3 * t + m];
+    config1.amr_vert_tris.assign(n, std::vector<int>());
+    for (int64_t i = 0; i < n; i++) config1.amr_vert_tris[i].assign(vt.begin() + stride * i, vt.begin() + stride * i + vtc[i]);
+    config1.amr_parent_verts.assign(config1.max_count, std::vector<int>(2, 0));
+    for (int64_t i = 0; i < n; i++) config1.amr_parent_verts[i] = {parent[2 * i], parent[2 * i + 1]};
+    config1.point_count = (int)n;
+    config1.tri_count = (int)T;
 }
 
 // helpers.h:362-372 (host, AoS stride 5)

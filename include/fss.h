@@ -210,6 +210,23 @@ fss_status fss_remesh(fss_ctx *ctx, const double *ax, const double *ay, const do
                       const double *atracer, const int32_t *tris, int64_t ntri, const int32_t *vt_off,
                       const int32_t *vt_idx, const double *x, const double *y, const double *z, double *vort,
                       double *tracer, int32_t *tri_out, int64_t n, int64_t *n_fallback, int64_t *n_missing);
+/* ---- adaptive mesh refinement (amr.h:13-178), host arrays ----
+ * One pass over the *ntri triangles present at the start, in index order: a
+ * triangle is split in four when its vertex vorticities span more than
+ * vorticity_tol or area * |w1 + w2 + w3| / 3 exceeds circulation_tol (the
+ * reference uses 0.0025 and 0.2 for 2562 particles).  Midpoints average the
+ * vertices' x, y, z, vort and tracer (tracer may be NULL) and are shared
+ * through the parent table parent[2 * p] = (lower, higher vertex), zero where
+ * unset; areas move area/6 from each corner to each midpoint; triangle i
+ * becomes (v1, v12, v31) and three triangles are appended; vert_tris (stride
+ * vt_stride, counts in vert_tri_count) is edited as the reference does.  The
+ * state, area, parent and list arrays must hold max_count points, tris
+ * tri_cap triangles.  Nothing happens once *n >= max_count (amr.h:22).
+ * *n_refined (may be NULL): triangles split. */
+fss_status fss_amr(int64_t *n, int64_t *ntri, int64_t max_count, int64_t tri_cap, double *x, double *y,
+                   double *z, double *vort, double *tracer, double *area, int32_t *tris, int32_t *parent,
+                   int32_t *vert_tris, int32_t *vert_tri_count, int vt_stride, double circulation_tol,
+                   double vorticity_tol, int64_t *n_refined);
 /* project_points (helpers.h:362-372) on device state. */
 fss_status fss_project_points(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
                               int64_t n);

@@ -1151,8 +1151,22 @@ ORC_API void orc_area_init(int64_t n, const double *x, const double *y, const do
 
 /* RK4 step with projection (intended semantics of new_dynamics.cpp:142-238,
  * section 3D): tree is the frozen t=0 tree or NULL for the direct sum. */
+static void rk4_step_impl(void *tree, int64_t n, double *x, double *y, double *z, double *vort,
+                          const double *area, double Omega, double dt, int nthreads, int project_last);
 ORC_API void orc_rk4_step(void *tree, int64_t n, double *x, double *y, double *z, double *vort,
                           const double *area, double Omega, double dt, int nthreads)
+{
+    rk4_step_impl(tree, n, x, y, z, vort, area, Omega, dt, nthreads, 1);
+}
+/* the step without the final projection: new_dynamics.cpp:211-218 adds
+ * c1234 to the state and refines (AMR) before projecting */
+ORC_API void orc_rk4_step_unprojected(void *tree, int64_t n, double *x, double *y, double *z, double *vort,
+                                      const double *area, double Omega, double dt, int nthreads)
+{
+    rk4_step_impl(tree, n, x, y, z, vort, area, Omega, dt, nthreads, 0);
+}
+static void rk4_step_impl(void *tree, int64_t n, double *x, double *y, double *z, double *vort,
+                          const double *area, double Omega, double dt, int nthreads, int project_last)
 {
     double *k[4], *st = (double *)malloc(sizeof(double) * 4 * n);
     for (int s = 0; s < 4; s++) k[s] = (double *)malloc(sizeof(double) * 4 * n);
@@ -1185,7 +1199,7 @@ ORC_API void orc_rk4_step(void *tree, int64_t n, double *x, double *y, double *z
             a *= dt / 6;
             ys[c][i] += a;
         }
-    orc_project_points(n, x, y, z, vort, Omega);
+    if (project_last) orc_project_points(n, x, y, z, vort, Omega);
     for (int s = 0; s < 4; s++) free(k[s]);
     free(st);
 }
@@ -1253,4 +1267,100 @@ ORC_API int64_t orc_remesh(int64_t n, const double *ax, const double *ay, const 
             tri_interp(tris + 3 * (int64_t)hit, ax, ay, az, avort, atr, t, c, Omega, &vort[i], tr ? &tr[i] : NULL);
     }
     return failure;
+}
+
+/* ------------------------------------------------------------------ */
+/* adaptive mesh refinement: amr.h:13-178                              */
+/* ------------------------------------------------------------------ */
+/* check_in_vec3 helpers.h:233-238: linear search of the parent table */
+static int32_t check_in_vec3(const int32_t *parent, int64_t count, int32_t v1, int32_t v2)
+{
+    for (int64_t i = 0; i < count; i++)
+        if (parent[2 * i] == v1 && parent[2 * i + 1] == v2) return (int32_t)i;
+    return -1;
+}
+
+/* replace helpers.h:290-297 */
+static void vt_replace(int32_t *l, int32_t cnt, int32_t find, int32_t repl)
+{
+    for (int32_t k = 0; k < cnt; k++)
+        if (l[k] == find) {
+            l[k] = repl;
+            break;
+        }
+}
+
+/* One refinement pass over the triangles present at its start (amr.h:23-171).
+ * state: 5 arrays (x, y, z, vort, tracer) of capacity max_count; parent:
+ * 2 * max_count (zeros where unset); vert_tris: stride 12 per point.  Returns
+ * the number of refined triangles (-1 when max_count points or 12 list
+ * entries would be exceeded); *n and *ntri are updated. */
+ORC_API int64_t orc_amr(int64_t *n_io, int64_t *ntri_io, int64_t max_count, double *x, double *y, double *z,
+                        double *vort, double *tracer, double *area, int32_t *tris, int32_t *parent,
+                        int32_t *vert_tris, int32_t *vert_tri_count, double circulation_tol, double vorticity_tol)
+{
+    int64_t n = *n_io, T = *ntri_io, T0 = *ntri_io, refined = 0;
+    if (n >= max_count) return 0;
+    double *st[5] = {x, y, z, vort, tracer};
+    for (int64_t i = 0; i < T0; i++) {
+        int32_t iv1 = tris[3 * i], iv2 = tris[3 * i + 1], iv3 = tris[3 * i + 2];
+        double vor1 = vort[iv1], vor2 = vort[iv2], vor3 = vort[iv3];
+        double max_val = fmax(vor1, fmax(vor2, vor3)), min_val = fmin(vor1, fmin(vor2, vor3));
+        double p1[3] = {x[iv1], y[iv1], z[iv1]}, p2[3] = {x[iv2], y[iv2], z[iv2]}, p3[3] = {x[iv3], y[iv3], z[iv3]};
+        double tri_area = sphere_tri_area(p1, p2, p3, 1.0);
+        double circulation = tri_area * fabs(vor1 + vor2 + vor3) / 3.0;
+        if (!(((max_val - min_val) > vorticity_tol) || (circulation > circulation_tol))) continue;
+        refined++;
+        area[iv1] -= tri_area / 6.0;
+        area[iv2] -= tri_area / 6.0;
+        area[iv3] -= tri_area / 6.0;
+        double v12[5], v23[5], v31[5];
+        for (int c = 0; c < 5; c++) {
+            double a = st[c] ? st[c][iv1] : 0, b = st[c] ? st[c][iv2] : 0, d = st[c] ? st[c][iv3] : 0;
+            v12[c] = (a + b) * 0.5;
+            v23[c] = (b + d) * 0.5;
+            v31[c] = (d + a) * 0.5;
+        }
+        int32_t iv12 = check_in_vec3(parent, max_count, iv1 < iv2 ? iv1 : iv2, iv1 < iv2 ? iv2 : iv1);
+        int32_t iv23 = check_in_vec3(parent, max_count, iv3 < iv2 ? iv3 : iv2, iv3 < iv2 ? iv2 : iv3);
+        int32_t iv31 = check_in_vec3(parent, max_count, iv1 < iv3 ? iv1 : iv3, iv1 < iv3 ? iv3 : iv1);
+        int32_t *ids[3] = {&iv12, &iv23, &iv31};
+        double *vals[3] = {v12, v23, v31};
+        int32_t pa[3][2] = {{iv1, iv2}, {iv2, iv3}, {iv1, iv3}};
+        for (int e = 0; e < 3; e++) {
+            if (*ids[e] == -1) {
+                if (n >= max_count) return -1; /* capacity: the reference writes out of bounds */
+                *ids[e] = (int32_t)n;
+                for (int c = 0; c < 5; c++)
+                    if (st[c]) st[c][n] = vals[e][c];
+                area[n] = tri_area / 6.0;
+                vert_tri_count[n] = 0;
+                int32_t a = pa[e][0], b = pa[e][1];
+                parent[2 * n] = a < b ? a : b;
+                parent[2 * n + 1] = a < b ? b : a;
+                n++;
+            } else {
+                area[*ids[e]] += tri_area / 6.0;
+            }
+        }
+        tris[3 * i] = iv1;
+        tris[3 * i + 1] = iv12;
+        tris[3 * i + 2] = iv31;
+        int32_t t2 = (int32_t)T, t3 = (int32_t)T + 1, t4 = (int32_t)T + 2;
+        int32_t nt[9] = {iv2, iv12, iv23, iv3, iv31, iv23, iv12, iv23, iv31};
+        for (int k = 0; k < 9; k++) tris[3 * T + k] = nt[k];
+        T += 3;
+        vt_replace(vert_tris + 12 * (int64_t)iv2, vert_tri_count[iv2], (int32_t)i, t2);
+        vt_replace(vert_tris + 12 * (int64_t)iv3, vert_tri_count[iv3], (int32_t)i, t3);
+        int32_t app[9][2] = {{iv12, (int32_t)i}, {iv12, t2}, {iv12, t4}, {iv23, t2}, {iv23, t3},
+                             {iv23, t4}, {iv31, (int32_t)i}, {iv31, t3}, {iv31, t4}};
+        for (int k = 0; k < 9; k++) {
+            int32_t v = app[k][0];
+            if (vert_tri_count[v] >= 12) return -1;
+            vert_tris[12 * (int64_t)v + vert_tri_count[v]++] = app[k][1];
+        }
+    }
+    *n_io = n;
+    *ntri_io = T;
+    return refined;
 }

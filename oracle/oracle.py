@@ -89,6 +89,11 @@ def lib():
     L.orc_remesh.restype = i64
     L.orc_remesh.argtypes = [i64, _dp, _dp, _dp, _dp, vp, _ip, i64, _ip, _ip, _dp, _dp, _dp, f64, _dp, vp,
                              _ip]
+    L.orc_amr.restype = i64
+    L.orc_amr.argtypes = [C.POINTER(i64), C.POINTER(i64), i64, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip,
+                          f64, f64]
+    L.orc_rk4_step_unprojected.restype = None
+    L.orc_rk4_step_unprojected.argtypes = [vp, i64, _dp, _dp, _dp, _dp, _dp, f64, f64, i32]
     L.orc_rk4_step.restype = None
     L.orc_rk4_step.argtypes = [vp, i64, _dp, _dp, _dp, _dp, _dp, f64, f64, i32]
     _LIB = L
@@ -279,13 +284,14 @@ def area_init(x, y, z, tris):
     return area
 
 
-def rk4_step(x, y, z, vort, area, dt, omega=2 * np.pi, tree=None, nthreads=None):
-    """One RK4 step with projection (new_dynamics.cpp:142-238 intent); returns new arrays."""
+def rk4_step(x, y, z, vort, area, dt, omega=2 * np.pi, tree=None, nthreads=None, project=True):
+    """One RK4 step with projection (new_dynamics.cpp:142-238 intent); returns new arrays.
+    project=False leaves out the final projection (the state AMR sees)."""
     a = [_c(v).copy() for v in (x, y, z, vort)]
     if nthreads is None:
         nthreads = os.cpu_count() or 1
-    lib().orc_rk4_step(tree.h if tree is not None else None, a[0].shape[0], *a, _c(area), omega, dt,
-                       nthreads)
+    f = lib().orc_rk4_step if project else lib().orc_rk4_step_unprojected
+    f(tree.h if tree is not None else None, a[0].shape[0], *a, _c(area), omega, dt, nthreads)
     return tuple(a)
 
 
@@ -316,3 +322,13 @@ def remesh(ax, ay, az, avort, tris, vt_off, vt_idx, x, y, z, omega=2 * np.pi, at
                           tris.shape[0] // 3, _c(vt_off, np.int32), _c(vt_idx, np.int32), _c(x), _c(y), _c(z),
                           omega, vort, tr.ctypes.data_as(C.c_void_p) if tr is not None else None, tri_out)
     return vort, tr, tri_out, int(nf)
+
+
+def amr(state, area, tris, parent, vert_tris, vert_tri_count, n, ntri, max_count,
+        circulation_tol=0.0025, vorticity_tol=0.2):
+    """amr.h:13-178 on host arrays edited in place (vert_tris stride 12);
+    returns (n, ntri, refined)."""
+    nn, nt = C.c_int64(n), C.c_int64(ntri)
+    r = lib().orc_amr(C.byref(nn), C.byref(nt), max_count, *state, area, tris.reshape(-1), parent.reshape(-1),
+                      vert_tris.reshape(-1), vert_tri_count, circulation_tol, vorticity_tol)
+    return nn.value, nt.value, int(r)

@@ -65,3 +65,77 @@ def test_driver_remesh_step_matches_oracle(orc, tmp_path, device):
     np.testing.assert_allclose(s1[:, :3], s0[:, :3], atol=1e-15)  # back on the grid
     assert rel_l2(s1[:, 3], want[3]) <= 1e-12
     assert rel_l2(s1[:, 4], want_t) <= 1e-12
+
+
+def _write_inputs(d, level):
+    """The reference's input files (read_points helpers.h:299-342) for the
+    level-`level` icosahedral grid with the rh4 vorticity of the driver."""
+    import fss
+    x, y, z, tris = fss.grid_icosahedral(level)
+    n = len(x)
+    off, idx = fss.vertex_triangles(n, tris)
+    lat = np.pi / 2 - np.arctan2(np.sqrt(x * x + y * y), z)
+    lon = np.arctan2(y, x)
+    vort = 2 * np.pi * np.sin(lat) / 7 + 30 * np.sin(lat) * np.cos(lat) ** 4 * np.cos(4 * lon)
+    np.savetxt(os.path.join(d, f"{n}points_rh4.csv"), np.stack([x, y, z, vort], 1), delimiter=",", fmt="%.17g")
+    np.savetxt(os.path.join(d, f"{n}tris.csv"), tris, delimiter=",", fmt="%d")
+    with open(os.path.join(d, f"{n}vert_tris.csv"), "w") as f:
+        for i in range(n):
+            f.write(",".join(str(v) for v in idx[off[i]:off[i + 1]]) + "\n")
+    np.savetxt(os.path.join(d, f"{n}vert_tri_count.csv"), np.diff(off), fmt="%d")
+
+
+def test_driver_reads_reference_input_files(tmp_path):
+    """-p: particles from {N}points_rh4.csv etc. give the same run as the
+    generated grid with the same field."""
+    ind = tmp_path / "in"
+    ind.mkdir()
+    _write_inputs(str(ind), 3)
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    a, _ = _run(tmp_path / "a", ["-x", "-c", "3", "-p", str(ind)])
+    b, _ = _run(tmp_path / "b", ["-x", "-c", "3"])
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("device", [False, True])
+def test_driver_amr_step_matches_oracle(orc, tmp_path, device):
+    """-a -b 1 (new_dynamics.cpp:218-225): after the step the mesh is refined
+    by the oracle's amr restatement on the stepped state, then projected.  The
+    host path refines the unprojected step as the reference does; the device
+    step (-g) projects inside its fused update."""
+    import oracle
+    import fss
+    args = ["-x", "-c", "3", "-i", "gv", "-a", "-b", "1"] + (["-g"] if device else [])
+    data, name = _run(tmp_path, args)
+    n = 10 * 4 ** 3 + 2
+    assert "_amr_" in name
+    s0 = data[:n]
+    x, y, z, vort, tracer, area = s0.T
+    sx, sy, sz, sv = orc.rk4_step(x, y, z, vort, area, 0.01, project=device)
+    _, _, _, tris = fss.grid_icosahedral(3)
+    T = len(tris)
+    M = 10 * 4 ** 4 + 2
+    st = [np.zeros(M) for _ in range(5)]
+    for a_, b_ in zip(st, (sx, sy, sz, sv, tracer)):
+        a_[:n] = b_
+    ar = np.zeros(M)
+    ar[:n] = area
+    tr = np.zeros((4 * T, 3), dtype=np.int32)
+    tr[:T] = tris
+    parent = np.zeros((M, 2), dtype=np.int32)
+    vt = np.zeros((M, 12), dtype=np.int32)
+    vtc = np.zeros(M, dtype=np.int32)
+    off, idx = oracle.vertex_triangles(n, tris)
+    for i in range(n):
+        vtc[i] = off[i + 1] - off[i]
+        vt[i, :vtc[i]] = idx[off[i]:off[i + 1]]
+    n1, T1, r = oracle.amr(st, ar, tr, parent, vt, vtc, n, T, M)
+    assert r > 0 and n1 > n
+    px, py, pz, pv = orc.project_points(st[0][:n1], st[1][:n1], st[2][:n1], st[3][:n1])
+    s1 = data[n:]
+    assert s1.shape == (n1, 6)
+    for c, want in enumerate((px, py, pz, pv)):
+        assert rel_l2(s1[:, c], want) <= 1e-12, c
+    assert rel_l2(s1[:, 4], st[4][:n1]) <= 1e-12
+    assert rel_l2(s1[:, 5], ar[:n1]) <= 1e-12
