@@ -420,6 +420,14 @@ template <int R> struct Acc<float, R> {
 // ------------------------------------------------------------------
 // near field (class A): direct pairs, R targets per lane
 // ------------------------------------------------------------------
+// The item's source ranges form one stream; it is staged into LDS in chunks
+// of NCH records.  Up to 64 ranges are read at once (one per lane) and
+// prefix-summed over the wave; every lane then stages the chunk slots lane,
+// lane + 64, ... (range found by binary search over the prefix sums), so the
+// record loads of a chunk are independent and issued together.  Ranges that
+// may hold the target itself (self ranges) form a second stream of their own
+// that stages the particle ids too and runs a masked evaluation skipping the
+// self pair; the other ranges run the branch-free loop.
 template <typename Real, int R>
 __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
                                              const Range *__restrict__ ranges,
@@ -432,8 +440,12 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
                                              const double *__restrict__ z, int64_t ostride, double *out)
 {
     using T4 = typename Rec<Real>::T4;
-    __shared__ T4 lds[NCH];
-    __shared__ double red[64][3 * R];
+    constexpr int LDS_BYTES = (sizeof(T4) * NCH > sizeof(double) * 64 * 3 * R) ? sizeof(T4) * NCH
+                                                                                 : sizeof(double) * 64 * 3 * R;
+    __shared__ __align__(32) unsigned char smem[LDS_BYTES];
+    __shared__ int32_t lid[NCH];
+    __shared__ int32_t r_pre[65], r_start[64];
+    T4 *lds = reinterpret_cast<T4 *>(smem);
     const NearItem it = items[blockIdx.x];
     const int lane = threadIdx.x;
     const int cnt = it.tgt_cnt;
@@ -463,34 +475,69 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
             pair_eval(xt[q], s, (Real)0, pq[0], pq[1], pq[2]);
         }
     };
-    auto flush = [&]() { A.flush(); };
-    int fill = 0;
-    staged_ranges(ranges, it.a_beg, it.a_end, src, recnode, p0, p, centres, ct, lds, ph, P, act, eval, flush,
-                  fill);
-    __syncthreads();
-    run_chunk(fill, ph, P, act, eval);
-    A.flush();
-    // ranges that may hold the target itself: skip the self pair by particle id
-    for (int r = it.a_beg; r < it.a_end; r++) {
-        Range rg = ranges[r];
-        if (!(rg.count & FSS_SELF_BIT)) continue;
-        int c0 = rg.count & ~FSS_SELF_BIT;
-        if (act)
-            for (int j = rg.start + ph; j < rg.start + c0; j += P) {
-                T4 s = to_frame(src[j], recnode[j], centres, ct);
-                int32_t sid = recid[j];
+    auto eval_masked = [&](int k) {
+        T4 s = lds[k];
+        int32_t sid = lid[k];
 #pragma unroll
-                for (int q = 0; q < R; q++) {
-                    Real *pq = A.part(q);
-                    if (sid != tid[q]) pair_eval(xt[q], s, (Real)0, pq[0], pq[1], pq[2]);
+        for (int q = 0; q < R; q++) {
+            Real *pq = A.part(q);
+            if (sid != tid[q]) pair_eval(xt[q], s, (Real)0, pq[0], pq[1], pq[2]);
+        }
+    };
+    for (int rb = it.a_beg; rb < it.a_end; rb += 64) {
+      const int nr = min(64, it.a_end - rb);
+      Range rg;
+      rg.start = rg.count = rg.node = rg.pad = 0;
+      if (lane < nr) rg = ranges[rb + lane];
+      const bool rself = (rg.count & FSS_SELF_BIT) != 0;
+      const bool any_self = __any(rself);
+      for (int pass = 0; pass < (any_self ? 2 : 1); pass++) {
+        // pass 0: ranges without the self pair; pass 1: self ranges, masked
+        const int c = (rself == (pass == 1)) ? (rg.count & ~FSS_SELF_BIT) : 0;
+        // inclusive scan of the range lengths over the wave
+        int incl = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        const int total = __shfl(incl, 63, 64);
+        __syncthreads(); // the previous batch is done with the range tables
+        r_pre[lane] = incl - c;
+        r_start[lane] = rg.start;
+        if (lane == 0) r_pre[64] = total;
+        __syncthreads();
+        for (int base = 0; base < total; base += NCH) {
+            const int m = min(NCH, total - base);
+#pragma unroll
+            for (int s = 0; s < NCH / 64; s++) {
+                const int k = lane + 64 * s;
+                if (k < m) {
+                    const int pos = base + k;
+                    int lo = 0, hi = nr - 1; // last range with r_pre <= pos
+                    while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (r_pre[mid] <= pos) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const int64_t j = (int64_t)r_start[lo] + (pos - r_pre[lo]);
+                    lds[k] = to_frame(src[j], src_node(j, p0, p, recnode), centres, ct);
+                    if (pass == 1) lid[k] = recid[j];
                 }
             }
-        A.flush();
+            __syncthreads();
+            if (pass == 1) run_chunk(m, ph, P, act, eval_masked);
+            else run_chunk(m, ph, P, act, eval);
+            A.flush();
+            __syncthreads();
+        }
+      }
     }
+    double *red = reinterpret_cast<double *>(smem); // [64][3R], after the last chunk
 #pragma unroll
     for (int q = 0; q < R; q++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) red[lane][3 * q + c] = A.total(q, c);
+        for (int c = 0; c < 3; c++) red[lane * 3 * R + 3 * q + c] = A.total(q, c);
     __syncthreads();
     if (ph == 0) {
 #pragma unroll
@@ -499,9 +546,10 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
             if (t >= cnt) continue;
             double a0 = 0, a1 = 0, a2 = 0;
             for (int h = 0; h < P; h++) {
-                a0 += red[h * T + tl][3 * q];
-                a1 += red[h * T + tl][3 * q + 1];
-                a2 += red[h * T + tl][3 * q + 2];
+                const double *rr = red + (h * T + tl) * 3 * R + 3 * q;
+                a0 += rr[0];
+                a1 += rr[1];
+                a2 += rr[2];
             }
             double xa[3];
             abs_pos(xt[q], tid[q], x, y, z, xa);
