@@ -45,7 +45,7 @@ template <class T> fss_status up(DevBuf &d, const std::vector<T> &h, hipStream_t
 struct Part {
     int L;
     int64_t n, nn, p0, n_charge, nslot_out;
-    std::vector<int32_t> recid, recnode, counts, inv, nodeof, node_slot, charge, oslot;
+    std::vector<int32_t> recid, recnode, counts, inv, nodeof, node_slot, charge, oslot, ipos;
     std::vector<uint8_t> rinfo;
     std::vector<int4> nrange;
     std::vector<double> tri_info, tri_xyz;
@@ -87,6 +87,7 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         FSS_TRY(down(P.recid, T.recid, T.p0, s));
         FSS_TRY(down(P.recnode, T.recnode, T.p0, s));
         FSS_TRY(down(P.rinfo, T.rinfo, T.p0, s));
+        FSS_TRY(down(P.ipos, T.ipos, T.n, s));
         FSS_TRY(down(P.oslot, T.oslot, T.nn, s));
         FSS_TRY(down(P.nrange, T.nrange, T.nn, s));
         FSS_TRY(down(P.counts, T.counts, T.nn, s));
@@ -146,7 +147,7 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         r.node = (int32_t)(r.node + nbase[k]);
         return r;
     };
-    std::vector<int32_t> recid, recnode, counts, inv, nodeof, node_slot, charge, oslot;
+    std::vector<int32_t> recid, recnode, counts, inv, nodeof, node_slot, charge, oslot, ipos;
     std::vector<uint8_t> rinfo;
     std::vector<int4> nrange;
     std::vector<double> tri_info, tri_xyz;
@@ -163,6 +164,7 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         for (int32_t v : P.recid) recid.push_back((int32_t)(v + pbase[k]));
         for (int32_t v : P.recnode) recnode.push_back((int32_t)(v + nbase[k]));
         rinfo.insert(rinfo.end(), P.rinfo.begin(), P.rinfo.end());
+        for (int32_t v : P.ipos) ipos.push_back((int32_t)(v + sbase[k]));
         for (int32_t v : P.oslot) oslot.push_back(v >= 0 ? (int32_t)(v + obase[k]) : -1);
         for (int4 r : P.nrange) {
             r.x = (int32_t)(r.x + sbase[k]);
@@ -239,6 +241,7 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
     FSS_TRY(up(T.recid, recid, s));
     FSS_TRY(up(T.recnode, recnode, s));
     FSS_TRY(up(T.rinfo, rinfo, s));
+    FSS_TRY(up(T.ipos, ipos, s));
     FSS_TRY(up(T.oslot, oslot, s));
     FSS_TRY(up(T.nrange, nrange, s));
     FSS_TRY(up(T.counts, counts, s));

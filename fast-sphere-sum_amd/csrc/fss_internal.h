@@ -172,6 +172,7 @@ struct Tree {
     DevBuf recid;    // int32 [p0]: particle of every record
     DevBuf recnode;  // int32 [p0]: node whose centre is the origin of the record (mixed precision)
     DevBuf rinfo;    // uint8 [p0]: main records: levels on the main path | 0x80 if exception members exist
+    DevBuf ipos;     // int32 [n]: main record of every particle
     DevBuf oslot;    // int32 [nn]: output slot of a node's first main target, or -1 (no near work)
     DevBuf nrange;   // int4 [nn]: {main start, main count, exception start, exception count}
     DevBuf inv_active; // int32 [L][n]: output slot of a particle in a class-A node, or -1
@@ -259,6 +260,7 @@ struct fss_ctx {
     fss::DevBuf phi;     // 3*p doubles per proxy work item (partial proxy values)
     fss::DevBuf alpha;   // 3*p doubles per interpolant slot
     fss::DevBuf upart;   // p doubles per partial moment of the upward pass
+    fss::DevBuf acc3;    // 3 doubles per record: per-particle sums in tree order
     int near_r = 0;      // targets per lane of the near kernel: 2, 4 or 0 (4 in FP64, 2 in FP32)
     fss::DevBuf rk_k;    // 4 stages x 4 fields x n
     fss::DevBuf rk_y;    // stage state 4 x n

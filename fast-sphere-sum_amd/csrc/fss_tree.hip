@@ -191,7 +191,7 @@ __global__ void k_main_keys(const int32_t *__restrict__ loc, const uint8_t *__re
 // path) and particle of every main record
 __global__ void k_main_ranges(const int32_t *__restrict__ perm, const int32_t *__restrict__ loc,
                               const uint8_t *__restrict__ found, int64_t n, int L, int4 *nrange,
-                              int32_t *recid, int32_t *recnode, uint8_t *rinfo)
+                              int32_t *recid, int32_t *recnode, uint8_t *rinfo, int32_t *ipos)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -215,6 +215,7 @@ __global__ void k_main_ranges(const int32_t *__restrict__ perm, const int32_t *_
     recid[i] = (int32_t)id;
     recnode[i] = deepest;
     rinfo[i] = (uint8_t)(e | (exc ? 0x80 : 0));
+    ipos[id] = (int32_t)i;
 }
 
 // exception memberships: (node, particle) of every level a particle is on a
@@ -682,7 +683,7 @@ void tree_release(Tree &t)
                       &t.recnode, &t.nrange,
                       &t.near_items, &t.proxy_items, &t.ranges_a,
                       &t.ranges_b, &t.charge_nodes, &t.proxy_xyz, &t.up_items, &t.up_multi, &t.tinv,
-                      &t.rinfo, &t.oslot, &t.plist};
+                      &t.rinfo, &t.oslot, &t.plist, &t.ipos};
     for (DevBuf *b : bufs) b->release();
     t.built = false;
 }
@@ -806,6 +807,7 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
         FSS_TRY(T.recid.ensure(sizeof(int32_t) * T.p0));
         FSS_TRY(T.recnode.ensure(sizeof(int32_t) * T.p0));
         FSS_TRY(T.rinfo.ensure(T.p0));
+        FSS_TRY(T.ipos.ensure(sizeof(int32_t) * n));
         FSS_HIP(hipMemsetAsync(T.rinfo.p, 0, T.p0, s));
         FSS_TRY(T.nrange.ensure(sizeof(int4) * T.nn));
         FSS_HIP(hipMemsetAsync(T.nrange.p, 0, sizeof(int4) * T.nn, s));
@@ -813,7 +815,7 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
                            T.counts.as<int32_t>(), ecnt, T.nn, T.nrange.as<int4>());
         hipLaunchKernelGGL(k_main_ranges, dim3(gp), dim3(256), 0, s, v1.as<int32_t>(), T.loc.as<int32_t>(),
                            T.found.as<uint8_t>(), n, L, T.nrange.as<int4>(), T.recid.as<int32_t>(),
-                           T.recnode.as<int32_t>(), T.rinfo.as<uint8_t>());
+                           T.recnode.as<int32_t>(), T.rinfo.as<uint8_t>(), T.ipos.as<int32_t>());
         if (ne > 0) {
             // exception memberships, sorted by (node, particle)
             DevBuf ek0, ek1, ev0, ev1;
