@@ -524,6 +524,16 @@ fss_status fss_sal_mlp_train(fss_ctx *ctx, int steps, int64_t samples, double lr
     })
 }
 
+fss_status fss_sal_tree(fss_ctx *ctx, const fss_particles *p, int kind, double *out)
+{
+    FSS_GUARD({
+        if (!ctx || !p || !p->x || !p->y || !p->z || !p->vort || !p->area || !out)
+            return set_error("null argument"), FSS_ERR_INVALID;
+        FSS_HIP(hipSetDevice(ctx->cfg.device));
+        return sal_tree(ctx, p, kind, out);
+    })
+}
+
 fss_status fss_sal_mlp_params(fss_ctx *ctx, float *params, int set)
 {
     if (!ctx || !params) return set_error("null argument"), FSS_ERR_INVALID;

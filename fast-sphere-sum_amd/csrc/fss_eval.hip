@@ -1019,6 +1019,46 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
     return FSS_OK;
 }
 
+// FP64 source records of a stage (tree order + proxies) with the BLTC
+// modified charges, for other kernels on the same plan (the SAL treecode)
+template <int D> static fss_status sources64_degree(fss_ctx *ctx, const fss_particles *pp)
+{
+    Tree &T = ctx->tree;
+    hipStream_t s = ctx->stream;
+    constexpr int p = npts<D>();
+    const int64_t p0 = T.p0, nprox = T.nn * p, nsrc = p0 + nprox;
+    FSS_TRY(ctx->src64.ensure(sizeof(double4) * nsrc));
+    FSS_TRY(ctx->upart.ensure(sizeof(double) * p * std::max<int64_t>(1, T.n_up_partial)));
+    double4 *src = ctx->src64.as<double4>();
+    hipLaunchKernelGGL(k_gather64, dim3((unsigned)div_up(nsrc, 256)), dim3(256), 0, s, pp->x, pp->y, pp->z, pp->vort,
+                       pp->area, T.recid.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, src);
+    if (T.n_up > 0) {
+        hipLaunchKernelGGL((k_upward<double, D>), dim3((unsigned)T.n_up), dim3(64), 0, s, T.up_items.as<int4>(),
+                           T.recnode.as<int32_t>(), T.tri_info.as<double>(), T.tinv.as<double>(),
+                           ctx->d_vinv.as<double>(), p0, src, ctx->upart.as<double>());
+        if (T.n_up_multi > 0)
+            hipLaunchKernelGGL((k_upward_finish<double, D>), dim3((unsigned)T.n_up_multi), dim3(64), 0, s,
+                               T.up_multi.as<int4>(), ctx->upart.as<double>(), ctx->d_vinv.as<double>(), p0, src);
+    }
+    FSS_HIP(hipGetLastError());
+    return FSS_OK;
+}
+
+fss_status tree_sources64(fss_ctx *ctx, const fss_particles *pp)
+{
+    switch (ctx->interp.degree) {
+    case 1: return sources64_degree<1>(ctx, pp);
+    case 2: return sources64_degree<2>(ctx, pp);
+    case 3: return sources64_degree<3>(ctx, pp);
+    case 4: return sources64_degree<4>(ctx, pp);
+    case 5: return sources64_degree<5>(ctx, pp);
+    case 6: return sources64_degree<6>(ctx, pp);
+    case 7: return sources64_degree<7>(ctx, pp);
+    case 8: return sources64_degree<8>(ctx, pp);
+    default: set_error("unsupported degree"); return FSS_ERR_INVALID;
+    }
+}
+
 fss_status velocity_tree(fss_ctx *ctx, const fss_particles *pp, fss_velocity *out)
 {
     Tree &T = ctx->tree;

@@ -305,6 +305,11 @@ fss_status rk4_run_graph(fss_ctx *ctx, double *x, double *y, double *z, double *
                          const double *area, int64_t n, double dt, int use_tree, int steps);
 fss_status project_points(fss_ctx *ctx, double *x, double *y, double *z, double *vort, int64_t n);
 
+// FP64 records + BLTC charges of the current tree for a stage (fss_eval.hip)
+fss_status tree_sources64(fss_ctx *ctx, const fss_particles *pp);
+// SAL convolution by the treecode (fss_sal_tree.hip)
+fss_status sal_tree(fss_ctx *ctx, const fss_particles *pp, int kind, double *out);
+
 // ---- remesh (fss_remesh.hip) ----
 fss_status remesh(fss_ctx *ctx, const double *ax, const double *ay, const double *az, const double *avort,
                   const double *atr, const int32_t *tris, int64_t ntri, const int32_t *vt_off,

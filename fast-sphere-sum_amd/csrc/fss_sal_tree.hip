@@ -1,0 +1,339 @@
+// fss_sal_tree.hip -- SAL convolution by the barycentric-Lagrange treecode.
+//
+// eta_SAL(x_i) = sum_j G(s_ij) eta_j A_j, j = i included (salpotential,
+// SAL/GFnewtest.ipynb:9934-9949), s = |x_i - x_j| / 2, on the same tree,
+// interaction list and evaluation plan as the velocity (BLTC mode): near
+// items evaluate G directly, proxy items evaluate G between the target
+// node's proxies and its class-B sources (particles, or source proxies that
+// carry the modified charges of the upward pass), the interpolant of every
+// target node is solved and evaluated at its particles.  Scalar kernel, FP64
+// records; G is one of
+//   KIND 0  the tabulated series (cubic interpolation, the fast fallback),
+//   KIND 1  the learned kernel (analytic singular part + network),
+//   KIND 2  the series itself by Clenshaw summation (the judge).
+#include <hip/hip_runtime.h>
+
+#include "fss_geom.h"
+#include "fss_sal_dev.h"
+
+namespace fss {
+
+namespace {
+
+constexpr int NCH_S = 256;
+
+template <int D> constexpr int npts_s() { return (D + 1) * (D + 2) / 2; }
+
+struct KernelArgs {
+    const double *tab;
+    int64_t M;
+    const float *P;
+    double A, mu, sigma, g0;
+    float log_smin, inv_span;
+    const double *c;
+    int N;
+};
+
+template <int KIND> __device__ inline double gfun(const KernelArgs &k, double s)
+{
+    if (KIND == 0) return table_eval(k.tab, k.M, s);
+    if (KIND == 1) return mlp_kernel(k.P, s, k.A, k.mu, k.sigma, k.g0, k.log_smin, k.inv_span);
+    return legendre_sum(k.c, k.N, 1.0 - 2.0 * s * s);
+}
+
+__device__ inline double half_dist(const double4 &a, const double4 &b)
+{
+    double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return 0.5 * sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// near field: one wave per near item, one lane per (target, phase); every
+// source range of the item (the self ranges too: j = i is part of the sum)
+// staged in LDS chunks
+template <int KIND>
+__global__ void __launch_bounds__(64) k_sal_near(const NearItem *__restrict__ items, const Range *__restrict__ ranges,
+                                                 const double4 *__restrict__ src, KernelArgs ka, double *out)
+{
+    __shared__ double4 lds[NCH_S];
+    __shared__ double red[64];
+    const NearItem it = items[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int cnt = it.tgt_cnt;
+    const int P = 64 / cnt;
+    const int ph = lane / cnt, tl = lane - ph * cnt;
+    const bool act = ph < P;
+    const double4 xt = src[it.tgt_start + (act ? tl : 0)];
+    double acc = 0.0;
+    int fill = 0;
+    auto run = [&]() {
+        if (act)
+            for (int k = ph; k < fill; k += P) {
+                const double4 s = lds[k];
+                acc = fma(gfun<KIND>(ka, half_dist(xt, s)), s.w, acc);
+            }
+    };
+    for (int r = it.a_beg; r < it.a_end; r++) {
+        const Range rg = ranges[r];
+        const int c = rg.count & ~FSS_SELF_BIT;
+        for (int base = 0; base < c; base += 64) {
+            const int m = min(64, c - base);
+            if (fill + m > NCH_S) {
+                __syncthreads();
+                run();
+                __syncthreads();
+                fill = 0;
+            }
+            if (lane < m) lds[fill + lane] = src[rg.start + base + lane];
+            fill += m;
+        }
+    }
+    __syncthreads();
+    run();
+    red[lane] = acc;
+    __syncthreads();
+    if (ph == 0 && act) {
+        double a = 0.0;
+        for (int h = 0; h < P; h++) a += red[h * cnt + tl];
+        out[it.out_base + tl] = a;
+    }
+}
+
+// proxy values: one lane per (item, proxy), the lane walks the item's ranges
+template <int KIND, int D>
+__global__ void __launch_bounds__(64) k_sal_proxy(const ProxyItem *__restrict__ items, int64_t nitems,
+                                                  const Range *__restrict__ ranges, const double4 *__restrict__ src,
+                                                  int64_t p0, KernelArgs ka, double *phi)
+{
+    constexpr int PP = npts_s<D>();
+    constexpr int IPW = 64 / PP;
+    const int lane = threadIdx.x;
+    const int il = lane / PP, k = lane - il * PP;
+    const int64_t item = (int64_t)blockIdx.x * IPW + il;
+    if (il >= IPW || item >= nitems) return;
+    const ProxyItem it = items[item];
+    const double4 yk = src[p0 + (int64_t)it.node * PP + k];
+    double acc = 0.0;
+    for (int r = it.b_beg; r < it.b_end; r++) {
+        const Range rg = ranges[r];
+        const int c = rg.count & ~FSS_SELF_BIT;
+        for (int j = 0; j < c; j++) {
+            const double4 s = src[rg.start + j];
+            acc = fma(gfun<KIND>(ka, half_dist(yk, s)), s.w, acc);
+        }
+    }
+    phi[item * PP + k] = acc;
+}
+
+// sum the chunks of every interpolant slot in order and solve V alpha = phi
+// (one right-hand side)
+template <int D>
+__global__ void k_sal_solve(const ProxySlot *__restrict__ slots, int64_t nslot, const double *__restrict__ phi,
+                            const double *__restrict__ lu, const int32_t *__restrict__ ipiv, double *alpha)
+{
+    constexpr int P = npts_s<D>();
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nslot) return;
+    const ProxySlot sl = slots[q];
+    double b[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) b[k] = 0.0;
+    for (int part = sl.part_beg; part < sl.part_end; part++)
+#pragma unroll
+        for (int k = 0; k < P; k++) b[k] += phi[(int64_t)part * P + k];
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+        int r = ipiv[k];
+        if (r != k) {
+            double bk = b[k], br = 0.0;
+#pragma unroll
+            for (int t = 0; t < P; t++)
+                if (t == r) br = b[t];
+#pragma unroll
+            for (int t = 0; t < P; t++)
+                if (t == r) b[t] = bk;
+            b[k] = br;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < P; k++)
+#pragma unroll
+        for (int r = k + 1; r < P; r++) b[r] = b[r] - b[k] * lu[r + k * P];
+#pragma unroll
+    for (int k = P - 1; k >= 0; k--) {
+        b[k] = b[k] / lu[k + k * P];
+#pragma unroll
+        for (int r = 0; r < k; r++) b[r] = b[r] - b[k] * lu[r + k * P];
+    }
+    for (int k = 0; k < P; k++) alpha[q * P + k] = b[k];
+}
+
+template <int D>
+__device__ inline double sal_interp(const double *__restrict__ tinv, const double *__restrict__ alpha, int32_t g,
+                                    int32_t sl, const double *pt)
+{
+    constexpr int P = npts_s<D>();
+    const double *R = tinv + 9 * (int64_t)g;
+    double c[3];
+    for (int m = 0; m < 3; m++) c[m] = fma(R[3 * m], pt[0], fma(R[3 * m + 1], pt[1], R[3 * m + 2] * pt[2]));
+    const double is = 1.0 / (c[0] + c[1] + c[2]);
+    const double s = c[0] * is, t = c[1] * is;
+    double sp[D + 1], tp[D + 1];
+    sp[0] = tp[0] = 1.0;
+    for (int k = 1; k <= D; k++) {
+        sp[k] = sp[k - 1] * s;
+        tp[k] = tp[k - 1] * t;
+    }
+    const double *al = alpha + (int64_t)sl * P;
+    double v = 0.0;
+    for (int i = 0; i <= D; i++)
+        for (int j = 0; j <= i; j++) v = fma(sp[i - j] * tp[j], al[i * (i + 1) / 2 + j], v);
+    return v;
+}
+
+// per main record (tree order): near slots of its main path + interpolants of
+// the nodes on it; memberships outside the main path from the per-level maps
+template <int D>
+__global__ void k_sal_combine(const int32_t *__restrict__ recid, const int32_t *__restrict__ recnode,
+                              const uint8_t *__restrict__ rinfo, const int4 *__restrict__ nrange,
+                              const int32_t *__restrict__ oslot, const int32_t *__restrict__ node_slot,
+                              const int32_t *__restrict__ nodeof, const int32_t *__restrict__ inv_active,
+                              const double *__restrict__ alpha, const double *__restrict__ tinv,
+                              const double4 *__restrict__ src, const Instance *__restrict__ inst, int ninst,
+                              int64_t nrec, const double *__restrict__ out, double *acc)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrec) return;
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (inst[mid].rbase <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const Instance I = inst[lo];
+    if (i - I.rbase >= I.n) return;
+    const double4 r = src[i];
+    const double pt[3] = {r.x, r.y, r.z};
+    const int info = rinfo[i];
+    const int e = info & 0x7f;
+    double a = 0.0;
+    if (e > 0) {
+        const int ld = e - 1;
+        const int64_t locd = recnode[i] - I.nbase - fssg::level_offset(ld);
+        for (int l = 0; l < e; l++) {
+            const int32_t g = (int32_t)(I.nbase + fssg::level_offset(l) + (locd >> (2 * (ld - l))));
+            const int32_t os = oslot[g];
+            if (os >= 0) a += out[os + (i - nrange[g].x)];
+            const int32_t sl = node_slot[g];
+            if (sl >= 0) a += sal_interp<D>(tinv, alpha, g, sl, pt);
+        }
+    }
+    if (info & 0x80) {
+        const int64_t li = recid[i] - I.pbase;
+        for (int l = e; l < I.L; l++) {
+            const int64_t at = I.loff + (int64_t)l * I.n + li;
+            const int32_t g = nodeof[at];
+            if (g < 0) continue;
+            const int32_t pos = inv_active[at];
+            if (pos >= 0) a += out[pos];
+            const int32_t sl = node_slot[g];
+            if (sl >= 0) a += sal_interp<D>(tinv, alpha, g, sl, pt);
+        }
+    }
+    acc[i] = a;
+}
+
+__global__ void k_sal_scatter(const int32_t *__restrict__ ipos, int64_t n, const double *__restrict__ acc,
+                              double *out)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) out[id] = acc[ipos[id]];
+}
+
+template <int KIND, int D> fss_status run(fss_ctx *ctx, const KernelArgs &ka, double *result)
+{
+    Tree &T = ctx->tree;
+    hipStream_t s = ctx->stream;
+    constexpr int p = npts_s<D>();
+    const int64_t p0 = T.p0;
+    const double4 *src = ctx->src64.as<double4>();
+    FSS_TRY(ctx->out3.ensure(sizeof(double) * std::max<int64_t>(1, T.nslot_out)));
+    FSS_TRY(ctx->phi.ensure(sizeof(double) * p * std::max<int64_t>(1, T.n_proxy)));
+    FSS_TRY(ctx->alpha.ensure(sizeof(double) * p * std::max<int64_t>(1, T.n_slot)));
+    FSS_TRY(ctx->acc3.ensure(sizeof(double) * std::max<int64_t>(1, p0)));
+    if (T.n_near > 0)
+        hipLaunchKernelGGL((k_sal_near<KIND>), dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
+                           T.ranges_a.as<Range>(), src, ka, ctx->out3.as<double>());
+    if (T.n_proxy > 0)
+        hipLaunchKernelGGL((k_sal_proxy<KIND, D>), dim3((unsigned)div_up(T.n_proxy, 64 / p)), dim3(64), 0, s,
+                           T.proxy_items.as<ProxyItem>(), T.n_proxy, T.ranges_b.as<Range>(), src, p0, ka,
+                           ctx->phi.as<double>());
+    if (T.n_slot > 0)
+        hipLaunchKernelGGL((k_sal_solve<D>), dim3((unsigned)div_up(T.n_slot, 128)), dim3(128), 0, s,
+                           T.proxy_slots.as<ProxySlot>(), T.n_slot, ctx->phi.as<double>(), ctx->d_lu.as<double>(),
+                           ctx->d_ipiv.as<int32_t>(), ctx->alpha.as<double>());
+    hipLaunchKernelGGL((k_sal_combine<D>), dim3((unsigned)div_up(p0, 256)), dim3(256), 0, s, T.recid.as<int32_t>(),
+                       T.recnode.as<int32_t>(), T.rinfo.as<uint8_t>(), T.nrange.as<int4>(), T.oslot.as<int32_t>(),
+                       T.node_slot.as<int32_t>(), T.nodeof.as<int32_t>(), T.inv_active.as<int32_t>(),
+                       ctx->alpha.as<double>(), T.tinv.as<double>(), src, T.inst.as<Instance>(), T.ninst, p0,
+                       ctx->out3.as<double>(), ctx->acc3.as<double>());
+    hipLaunchKernelGGL(k_sal_scatter, dim3((unsigned)div_up(T.n, 256)), dim3(256), 0, s, T.ipos.as<int32_t>(), T.n,
+                       ctx->acc3.as<double>(), result);
+    FSS_HIP(hipGetLastError());
+    return FSS_OK;
+}
+
+template <int KIND> fss_status run_kind(fss_ctx *ctx, const KernelArgs &ka, double *result)
+{
+    switch (ctx->interp.degree) {
+    case 1: return run<KIND, 1>(ctx, ka, result);
+    case 2: return run<KIND, 2>(ctx, ka, result);
+    case 3: return run<KIND, 3>(ctx, ka, result);
+    case 4: return run<KIND, 4>(ctx, ka, result);
+    case 5: return run<KIND, 5>(ctx, ka, result);
+    default: set_error("sal treecode: degree 1..5"); return FSS_ERR_INVALID;
+    }
+}
+
+} // namespace
+
+fss_status sal_tree(fss_ctx *ctx, const fss_particles *pp, int kind, double *out)
+{
+    Tree &T = ctx->tree;
+    Sal &S = ctx->sal;
+    if (!T.built) return set_error("sal treecode: build the tree first"), FSS_ERR_STATE;
+    if (ctx->cfg.eval_mode != FSS_EVAL_BLTC) return set_error("sal treecode: needs eval_mode BLTC"), FSS_ERR_STATE;
+    if (pp->n != T.n) return set_error("particle count differs from the tree's"), FSS_ERR_INVALID;
+    if (!S.c.p) return set_error("sal: series not set"), FSS_ERR_STATE;
+    if (kind == 1 && !S.has_mlp) return set_error("sal: no network"), FSS_ERR_STATE;
+    if (kind < 0 || kind > 2) return set_error("sal: kind 0 (table), 1 (learned) or 2 (series)"), FSS_ERR_INVALID;
+    KernelArgs ka;
+    ka.tab = S.tab.as<double>();
+    ka.M = S.M;
+    ka.P = S.P.as<float>();
+    ka.A = S.A;
+    ka.mu = S.mu;
+    ka.sigma = S.sigma;
+    ka.g0 = S.g0;
+    ka.log_smin = (float)std::log(S.s_min);
+    ka.inv_span = (float)(1.0 / -std::log(S.s_min));
+    ka.c = S.c.as<double>();
+    ka.N = S.nmax;
+    hipEvent_t *ev = nullptr;
+    if (!ctx->capturing) {
+        timing_begin(ctx->timing);
+        ev = ctx->timing.slot();
+        FSS_HIP(hipEventRecord(ev[0], ctx->stream));
+    }
+    FSS_TRY(tree_sources64(ctx, pp)); // records w = eta A and the modified charges
+    if (ev) FSS_HIP(hipEventRecord(ev[1], ctx->stream));
+    fss_status st = kind == 0 ? run_kind<0>(ctx, ka, out) : kind == 1 ? run_kind<1>(ctx, ka, out)
+                                                                    : run_kind<2>(ctx, ka, out);
+    if (ev) {
+        FSS_HIP(hipEventRecord(ev[2], ctx->stream));
+        FSS_HIP(hipEventRecord(ev[3], ctx->stream));
+        ctx->timing.valid = true;
+    }
+    return st;
+}
+
+} // namespace fss

@@ -108,6 +108,7 @@ SIGNATURES = {
     "fss_sal_kernel": (_int, [_vp, _int, _vp, _i64, _vp]),
     "fss_sal_direct": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _int, _vp]),
     "fss_sal_mlp_nparam": (_int, []),
+    "fss_sal_tree": (_int, [_vp, C.POINTER(Particles), _int, _vp]),
     "fss_sal_mlp_init": (_int, [_vp, _u64]),
     "fss_sal_mlp_train": (_int, [_vp, _int, _i64, _dbl, _dbl, _u64, C.POINTER(_dbl)]),
     "fss_sal_mlp_params": (_int, [_vp, _vp, _int]),
@@ -447,6 +448,16 @@ class Context:
         out = torch.empty(m, dtype=torch.float64, device=self.device)
         _check(lib().fss_sal_direct(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), _dev_ptr(eta),
                                     _dev_ptr(area), n, ip, m, kind, _dev_ptr(out)))
+        return out
+
+    def sal_tree(self, x, y, z, eta, area, kind=0, out=None):
+        """SAL convolution at every particle by the treecode (tree built
+        with eval_mode BLTC on these particles)."""
+        import torch
+        p = self._particles(x, y, z, eta, area)
+        if out is None:
+            out = torch.empty(x.numel(), dtype=torch.float64, device=x.device)
+        _check(lib().fss_sal_tree(self.h, C.byref(p), int(kind), _dev_ptr(out)))
         return out
 
     def sal_mlp_init(self, seed=0):

@@ -301,6 +301,12 @@ fss_status fss_sal_kernel(fss_ctx *ctx, int kind, const double *s, int64_t m, do
 fss_status fss_sal_direct(fss_ctx *ctx, const double *x, const double *y, const double *z,
                           const double *eta, const double *area, int64_t n, const int64_t *idx,
                           int64_t m, int kind, double *out);
+/* SAL convolution by the treecode (no reference counterpart: the reference
+ * has only the direct sum salpotential, GFnewtest.ipynb:9934-9949): the
+ * context's tree must be built on the SAL particles with eval_mode BLTC;
+ * p->vort holds eta, p->area the areas (device); out[n] (device).  kind 0:
+ * tabulated series, 1: learned kernel, 2: the series by Clenshaw summation. */
+fss_status fss_sal_tree(fss_ctx *ctx, const fss_particles *p, int kind, double *out);
 /* Learned kernel: MLP 1 -> 64 -> 64 -> 1 (tanh), FP32 parameters in the order
  * W1[64], b1[64], W2[64][64] (row j = output unit), b2[64], w3[64], b3. */
 int fss_sal_mlp_nparam(void);
