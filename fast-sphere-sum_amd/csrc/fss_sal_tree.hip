@@ -34,11 +34,19 @@ struct KernelArgs {
     int N;
 };
 
-template <int KIND> __device__ inline double gfun(const KernelArgs &k, double s)
+// Ps: the network parameters staged in LDS (KIND 1)
+template <int KIND> __device__ inline double gfun(const KernelArgs &k, const float *Ps, double s)
 {
     if (KIND == 0) return table_eval(k.tab, k.M, s);
-    if (KIND == 1) return mlp_kernel(k.P, s, k.A, k.mu, k.sigma, k.g0, k.log_smin, k.inv_span);
+    if (KIND == 1) return mlp_kernel(Ps, s, k.A, k.mu, k.sigma, k.g0, k.log_smin, k.inv_span);
     return legendre_sum(k.c, k.N, 1.0 - 2.0 * s * s);
+}
+
+template <int KIND> __device__ inline void stage_params(const KernelArgs &k, float *Ps)
+{
+    if (KIND == 1)
+        for (int q = threadIdx.x; q < NPARAM; q += blockDim.x) Ps[q] = k.P[q];
+    __syncthreads();
 }
 
 __device__ inline double half_dist(const double4 &a, const double4 &b)
@@ -56,6 +64,8 @@ __global__ void __launch_bounds__(64) k_sal_near(const NearItem *__restrict__ it
 {
     __shared__ double4 lds[NCH_S];
     __shared__ double red[64];
+    __shared__ float Ps[KIND == 1 ? NPARAM : 1];
+    stage_params<KIND>(ka, Ps);
     const NearItem it = items[blockIdx.x];
     const int lane = threadIdx.x;
     const int cnt = it.tgt_cnt;
@@ -69,7 +79,7 @@ __global__ void __launch_bounds__(64) k_sal_near(const NearItem *__restrict__ it
         if (act)
             for (int k = ph; k < fill; k += P) {
                 const double4 s = lds[k];
-                acc = fma(gfun<KIND>(ka, half_dist(xt, s)), s.w, acc);
+                acc = fma(gfun<KIND>(ka, Ps, half_dist(xt, s)), s.w, acc);
             }
     };
     for (int r = it.a_beg; r < it.a_end; r++) {
@@ -106,6 +116,8 @@ __global__ void __launch_bounds__(64) k_sal_proxy(const ProxyItem *__restrict__ 
 {
     constexpr int PP = npts_s<D>();
     constexpr int IPW = 64 / PP;
+    __shared__ float Ps[KIND == 1 ? NPARAM : 1];
+    stage_params<KIND>(ka, Ps);
     const int lane = threadIdx.x;
     const int il = lane / PP, k = lane - il * PP;
     const int64_t item = (int64_t)blockIdx.x * IPW + il;
@@ -118,7 +130,7 @@ __global__ void __launch_bounds__(64) k_sal_proxy(const ProxyItem *__restrict__ 
         const int c = rg.count & ~FSS_SELF_BIT;
         for (int j = 0; j < c; j++) {
             const double4 s = src[rg.start + j];
-            acc = fma(gfun<KIND>(ka, half_dist(yk, s)), s.w, acc);
+            acc = fma(gfun<KIND>(ka, Ps, half_dist(yk, s)), s.w, acc);
         }
     }
     phi[item * PP + k] = acc;
