@@ -851,15 +851,30 @@ __global__ void k_combine(const int32_t *__restrict__ recid, const int32_t *__re
         const int64_t locd = recnode[i] - I.nbase - fssg::level_offset(ld);
         for (int l = 0; l < e; l++) {
             const int32_t g = (int32_t)(I.nbase + fssg::level_offset(l) + (locd >> (2 * (ld - l))));
-            const int32_t os = oslot[g];
-            if (os >= 0) {
-                const int64_t pos = os + (i - nrange[g].x);
-                a0 += out[pos];
-                a1 += out[ostride + pos];
-                a2 += out[2 * ostride + pos];
+            // a node shared by the whole wave (the upper levels): its data is
+            // read once through scalar loads instead of 64 vector lanes
+            const int32_t gu = __builtin_amdgcn_readfirstlane(g);
+            if (__all(g == gu)) {
+                const int32_t os = oslot[gu];
+                if (os >= 0) {
+                    const int64_t pos = os + (i - nrange[gu].x);
+                    a0 += out[pos];
+                    a1 += out[ostride + pos];
+                    a2 += out[2 * ostride + pos];
+                }
+                const int32_t sl = node_slot[gu];
+                if (sl >= 0) add_interp<D>(tinv, alpha, gu, sl, pt, normalise, a0, a1, a2);
+            } else {
+                const int32_t os = oslot[g];
+                if (os >= 0) {
+                    const int64_t pos = os + (i - nrange[g].x);
+                    a0 += out[pos];
+                    a1 += out[ostride + pos];
+                    a2 += out[2 * ostride + pos];
+                }
+                const int32_t sl = node_slot[g];
+                if (sl >= 0) add_interp<D>(tinv, alpha, g, sl, pt, normalise, a0, a1, a2);
             }
-            const int32_t sl = node_slot[g];
-            if (sl >= 0) add_interp<D>(tinv, alpha, g, sl, pt, normalise, a0, a1, a2);
         }
     }
     if (info & 0x80) {

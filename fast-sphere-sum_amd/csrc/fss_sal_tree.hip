@@ -233,10 +233,18 @@ __global__ void k_sal_combine(const int32_t *__restrict__ recid, const int32_t *
         const int64_t locd = recnode[i] - I.nbase - fssg::level_offset(ld);
         for (int l = 0; l < e; l++) {
             const int32_t g = (int32_t)(I.nbase + fssg::level_offset(l) + (locd >> (2 * (ld - l))));
-            const int32_t os = oslot[g];
-            if (os >= 0) a += out[os + (i - nrange[g].x)];
-            const int32_t sl = node_slot[g];
-            if (sl >= 0) a += sal_interp<D>(tinv, alpha, g, sl, pt);
+            const int32_t gu = __builtin_amdgcn_readfirstlane(g);
+            if (__all(g == gu)) { // node shared by the wave: scalar loads
+                const int32_t os = oslot[gu];
+                if (os >= 0) a += out[os + (i - nrange[gu].x)];
+                const int32_t sl = node_slot[gu];
+                if (sl >= 0) a += sal_interp<D>(tinv, alpha, gu, sl, pt);
+            } else {
+                const int32_t os = oslot[g];
+                if (os >= 0) a += out[os + (i - nrange[g].x)];
+                const int32_t sl = node_slot[g];
+                if (sl >= 0) a += sal_interp<D>(tinv, alpha, g, sl, pt);
+            }
         }
     }
     if (info & 0x80) {
