@@ -292,7 +292,8 @@ fss_status fss_sal_coefficients(int nmax, const double *h, const double *k, doub
 fss_status fss_sal_set_series(fss_ctx *ctx, const double *c, int nmax, int64_t table_points,
                               double A);
 /* G at m separations s (device arrays): kind 0 = series (Clenshaw, FP64),
- * kind 1 = learned kernel A (1/(2s) - 1) + mu + sigma * MLP(2s - 1). */
+ * kind 1 = learned kernel A (1/(2s) - 1) + mu + sigma * MLP(x), x as for
+ * fss_sal_mlp_norm. */
 fss_status fss_sal_kernel(fss_ctx *ctx, int kind, const double *s, int64_t m, double *g);
 /* salpotential (GFnewtest.ipynb:9934-9949): out[r] = sum_j G(d(i_r, j)) eta_j A_j
  * including j = i (G at theta = 0 is the finite truncated series), for targets
