@@ -82,6 +82,19 @@ __device__ inline void pair_eval(const double4 &xt, const double4 &s, double, do
     S2 = fma(c, s.z - xt.z, S2);
 }
 
+// FP64 near field of the treecode (eval_mode BLTC): 1 - x.y = |x - y|^2 / 2 on
+// the unit sphere (particles and source proxies lie on it); S accumulates
+// w (x_j - x_i) / |x_j - x_i|^2 and the caller doubles it.  Three VALU fewer
+// than the reference form per pair and free of its cancellation.
+__device__ inline void pair_eval_sph(const double4 &xt, const double4 &s, double &S0, double &S1, double &S2)
+{
+    double d0 = s.x - xt.x, d1 = s.y - xt.y, d2 = s.z - xt.z;
+    double c = s.w * rcp64(fma(d0, d0, fma(d1, d1, d2 * d2)));
+    S0 = fma(c, d0, S0);
+    S1 = fma(c, d1, S1);
+    S2 = fma(c, d2, S2);
+}
+
 // FP32: records are offsets from a common node centre and carry the doubled
 // weight 2 w; 2 (1 - x.y) = |d|^2 + off2, where off2 = 1 - |y|^2 for a target
 // off the sphere (planar proxy), so c = 2 w / (|d|^2 + off2) = w / (1 - x.y).
@@ -94,6 +107,20 @@ __device__ inline void pair_eval(const float4 &xt, const float4 &s, float off2, 
     S0 = fmaf(c, dx, S0);
     S1 = fmaf(c, dy, S1);
     S2 = fmaf(c, dz, S2);
+}
+
+// near-field pair of k_near: FP32 records always use the offset form; FP64
+// records the reference form (SPH false) or the sphere form (SPH true)
+template <bool SPH>
+__device__ inline void near_pair(const double4 &xt, const double4 &s, double &S0, double &S1, double &S2)
+{
+    if (SPH) pair_eval_sph(xt, s, S0, S1, S2);
+    else pair_eval(xt, s, 0.0, S0, S1, S2);
+}
+template <bool SPH>
+__device__ inline void near_pair(const float4 &xt, const float4 &s, float &S0, float &S1, float &S2)
+{
+    pair_eval(xt, s, 0.f, S0, S1, S2);
 }
 
 // FP64 far field (well separated: no cancellation): y_k x sum c_j x_j
@@ -428,7 +455,7 @@ template <int R> struct Acc<float, R> {
 // may hold the target itself (self ranges) form a second stream of their own
 // that stages the particle ids too and runs a masked evaluation skipping the
 // self pair; the other ranges run the branch-free loop.
-template <typename Real, int R>
+template <typename Real, int R, bool SPH>
 __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
                                              const Range *__restrict__ ranges,
                                              const typename Rec<Real>::T4 *__restrict__ src,
@@ -472,7 +499,7 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
 #pragma unroll
         for (int q = 0; q < R; q++) {
             Real *pq = A.part(q);
-            pair_eval(xt[q], s, (Real)0, pq[0], pq[1], pq[2]);
+            near_pair<SPH>(xt[q], s, pq[0], pq[1], pq[2]);
         }
     };
     auto eval_masked = [&](int k) {
@@ -481,7 +508,7 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
 #pragma unroll
         for (int q = 0; q < R; q++) {
             Real *pq = A.part(q);
-            if (sid != tid[q]) pair_eval(xt[q], s, (Real)0, pq[0], pq[1], pq[2]);
+            if (sid != tid[q]) near_pair<SPH>(xt[q], s, pq[0], pq[1], pq[2]);
         }
     };
     for (int rb = it.a_beg; rb < it.a_end; rb += 64) {
@@ -514,12 +541,12 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
                 const int k = lane + 64 * s;
                 if (k < m) {
                     const int pos = base + k;
-                    int lo = 0, hi = nr - 1; // last range with r_pre <= pos
-                    while (lo < hi) {
-                        int mid = (lo + hi + 1) >> 1;
-                        if (r_pre[mid] <= pos) lo = mid;
-                        else hi = mid - 1;
-                    }
+                    // last range with r_pre <= pos, branch-free (entries from nr
+                    // on hold the total, which exceeds pos)
+                    int lo = 0;
+#pragma unroll
+                    for (int step = 32; step > 0; step >>= 1)
+                        if (r_pre[lo + step] <= pos) lo += step;
                     const int64_t j = (int64_t)r_start[lo] + (pos - r_pre[lo]);
                     lds[k] = to_frame(src[j], src_node(j, p0, p, recnode), centres, ct);
                     if (pass == 1) lid[k] = recid[j];
@@ -554,6 +581,11 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
             double xa[3];
             abs_pos(xt[q], tid[q], x, y, z, xa);
             double x0 = xa[0], x1 = xa[1], x2 = xa[2];
+            if (SPH) { // S holds half the sum (denominator |d|^2 = 2 (1 - x.y))
+                a0 *= 2.0;
+                a1 *= 2.0;
+                a2 *= 2.0;
+            }
             int64_t pos = it.out_base + t;
             out[pos] = x1 * a2 - x2 * a1;
             out[ostride + pos] = x2 * a0 - x0 * a2;
@@ -962,7 +994,9 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
         }
         if (ev) FSS_HIP(hipEventRecord(ev[1], s));
         if (T.n_near > 0) {
-            auto kn = ctx->near_r == 2 ? k_near<double, 2> : k_near<double, 4>; // default 4
+            // default 4 targets per lane; the treecode's near pairs use the sphere form
+            auto kn = bltc ? (ctx->near_r == 2 ? k_near<double, 2, true> : k_near<double, 4, true>)
+                           : (ctx->near_r == 2 ? k_near<double, 2, false> : k_near<double, 4, false>);
             hipLaunchKernelGGL(kn, dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
                                T.ranges_a.as<Range>(), src, T.recid.as<int32_t>(), T.recnode.as<int32_t>(), p0, p,
                                T.tri_info.as<double>(), pp->x, pp->y, pp->z, ost, ctx->out3.as<double>());
@@ -1001,7 +1035,7 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
         }
         if (ev) FSS_HIP(hipEventRecord(ev[1], s));
         if (T.n_near > 0) {
-            auto kn = ctx->near_r == 4 ? k_near<float, 4> : k_near<float, 2>; // default 2
+            auto kn = ctx->near_r == 4 ? k_near<float, 4, false> : k_near<float, 2, false>; // default 2
             hipLaunchKernelGGL(kn, dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
                                T.ranges_a.as<Range>(), src, T.recid.as<int32_t>(), T.recnode.as<int32_t>(), p0, p,
                                T.tri_info.as<double>(), pp->x, pp->y, pp->z, ost, ctx->out3.as<double>());

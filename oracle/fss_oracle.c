@@ -104,6 +104,21 @@ static void bve_gfunc(const double *x, const double *y, double *out)
     out[2] *= s;
 }
 
+/* Near pairs of the barycentric-Lagrange treecode (eval_mode BLTC, no
+ * reference counterpart): the same kernel with 1 - x.y written as
+ * |x - y|^2 / 2, equal on the unit sphere and free of the cancellation of
+ * 1 - x.y for close pairs. */
+static void bve_gfunc_sph(const double *x, const double *y, double *out)
+{
+    double d0 = y[0] - x[0], d1 = y[1] - x[1], d2 = y[2] - x[2];
+    double denom = 0.5 * (d0 * d0 + d1 * d1 + d2 * d2);
+    cross3(x, y, out);
+    double s = 1 / denom;
+    out[0] *= s;
+    out[1] *= s;
+    out[2] *= s;
+}
+
 /* helpers.h:165-177 */
 static void tri_sides(const double *p1, const double *p2, const double *p3, double *s)
 {
@@ -779,7 +794,8 @@ static void k_pp(double *mod, const orc_state *s, const orc_tree *t, const int32
             int32_t sj = sl[j];
             if (ti != sj) {
                 load_pt(s, sj, pj);
-                bve_gfunc(pi, pj, k);
+                if (t->eval_mode == EVAL_BLTC) bve_gfunc_sph(pi, pj, k);
+                else bve_gfunc(pi, pj, k);
                 double m = s->w_om[sj] * s->area[sj];
                 for (int c = 0; c < 3; c++) k[c] *= m;
                 for (int c = 0; c < 3; c++) pc[c] += k[c];
@@ -991,7 +1007,7 @@ static void k_bltc(double *mod, const orc_state *s, const orc_tree *t, const int
             load_pt(s, tl[i], pt);
             for (int m = 0; m < p; m++) {
                 sph_proxy(t, sv1, sv2, sv3, m, ym);
-                bve_gfunc(pt, ym, f);
+                bve_gfunc_sph(pt, ym, f);
                 for (int c = 0; c < 3; c++) acc[c] += f[c] * q[m];
             }
             for (int c = 0; c < 3; c++) mod[3 * (int64_t)tl[i] + c] += acc[c];

@@ -353,6 +353,19 @@ def test_tree_equals_direct_when_nothing_is_far(orc, grid5):
     assert rel_l2(np.stack(a[:3], 1), np.stack(b[:3], 1)) < 1e-13
 
 
+def test_bltc_near_pairs_equal_reference_kernel(orc, grid5):
+    # theta -> 0 in BLTC mode: every pair is a near pair in the sphere form
+    # |x - y|^2 / 2; on the unit sphere it equals the reference's 1 - x.y, so
+    # the sum agrees with the direct sum up to rounding (a dropped 1/2, a
+    # sign or a swapped operand would be off by O(1))
+    x, y, z, vort, area, _ = grid5
+    T = orc.Tree(x, y, z, 3, theta=1e-6, assign=1, eval_mode=orc.EVAL_BLTC, node_rule=1)
+    assert (T.interactions()[:, 6] == 0).all()
+    a = np.stack(T.rhs(vort, area, nthreads=4)[:3], 1)
+    b = np.stack(orc.rhs_direct(x, y, z, vort, area)[:3], 1)
+    assert rel_l2(a, b) < 1e-13
+
+
 def test_bltc_converges_with_degree(orc, grid5):
     x, y, z, vort, area, _ = grid5
     ref = np.stack(orc.rhs_direct(x, y, z, vort, area)[:3], 1)
