@@ -29,6 +29,7 @@ METRIC = "particle velocities/sec per RK4 stage at rel-L2≤1e-6 vs FP64 direct,
 PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 peak (published)
 FLOPS_PAIR_FP64 = 15        # 1 - x.y (5) + sub (1) + division (1) + w* (1) + 3 FMA (6)
+FLOPS_PAIR_FP64_SPH = 16    # treecode near pairs: d (3) + |d|^2 (5) + division (1) + w* (1) + 3 FMA (6)
 FLOPS_PAIR_FP32 = 17        # d (3) + |d|^2 (5) + rcp (1) + w* (1) + 1/2 (1) + 3 FMA (6)
 
 
@@ -434,7 +435,7 @@ def main():
                              f"bltc {W['headline']}", args.steps)
     spairs = self_pairs(ctx)
     alg_pairs = head["pairs_near"] - spairs
-    flp = FLOPS_PAIR_FP64 if hprec == fss.FP64 else FLOPS_PAIR_FP32
+    flp = FLOPS_PAIR_FP64_SPH if hprec == fss.FP64 else FLOPS_PAIR_FP32
     peak = PEAK_FP64_TFLOPS if hprec == fss.FP64 else PEAK_FP32_TFLOPS
     kname = "k_near<double,4>" if hprec == fss.FP64 else "k_near<float,2>"
     achieved = alg_pairs * flp / (head["near_kernel_ms"] * 1e-3) / 1e12

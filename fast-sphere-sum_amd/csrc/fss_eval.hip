@@ -123,6 +123,34 @@ __device__ inline void near_pair(const float4 &xt, const float4 &s, float &S0, f
     pair_eval(xt, s, 0.f, S0, S1, S2);
 }
 
+// The same pair with the self pair masked out (ranges that hold the target
+// itself): the weight is replaced by 0 after the division, so the self pair
+// (denominator 0 or rounding noise) adds exactly 0 -- a select, no branch.
+template <bool SPH>
+__device__ inline void near_pair_masked(const double4 &xt, const double4 &s, bool self, double &S0, double &S1,
+                                        double &S2)
+{
+    double d0 = s.x - xt.x, d1 = s.y - xt.y, d2 = s.z - xt.z;
+    double den = SPH ? fma(d0, d0, fma(d1, d1, d2 * d2)) : 1.0 - dot_ref(xt, s);
+    double c = s.w * rcp64(den);
+    c = self ? 0.0 : c;
+    S0 = fma(c, d0, S0);
+    S1 = fma(c, d1, S1);
+    S2 = fma(c, d2, S2);
+}
+template <bool SPH>
+__device__ inline void near_pair_masked(const float4 &xt, const float4 &s, bool self, float &S0, float &S1,
+                                        float &S2)
+{
+    float dx = s.x - xt.x, dy = s.y - xt.y, dz = s.z - xt.z;
+    float den = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, 0.f)));
+    float c = s.w * rcp32(den);
+    c = self ? 0.f : c;
+    S0 = fmaf(c, dx, S0);
+    S1 = fmaf(c, dy, S1);
+    S2 = fmaf(c, dz, S2);
+}
+
 // FP64 far field (well separated: no cancellation): y_k x sum c_j x_j
 __device__ inline void pair_far(const double4 &xt, const double4 &s, double, double &S0, double &S1,
                                 double &S2)
@@ -353,6 +381,15 @@ __device__ inline void run_chunk(int fill, int ph, int P, bool act, Eval eval)
     if (k < fill) eval(k);
 }
 
+// Evaluates the staged entries [lo, fill) owned by this lane (the same slot to
+// lane assignment as run_chunk: slot k belongs to phase k mod P).
+template <typename Eval>
+__device__ inline void run_chunk_from(int lo, int fill, int ph, int P, bool act, Eval eval)
+{
+    if (!act) return;
+    for (int k = ph < lo ? ph + (lo - ph + P - 1) / P * P : ph; k < fill; k += P) eval(k);
+}
+
 // Stages the sources of ranges [r0, r1) that are not flagged self into LDS in
 // chunks of NCH; after every full chunk the lane's entries are evaluated and
 // flush() moves the FP32 partial sums into FP64.  Records are moved to the
@@ -447,17 +484,15 @@ template <int R> struct Acc<float, R> {
 // ------------------------------------------------------------------
 // near field (class A): direct pairs, R targets per lane
 // ------------------------------------------------------------------
-// The item's source ranges form one stream; it is staged into LDS in chunks
-// of NCH records.  Up to 64 ranges are read at once (one per lane) and
-// prefix-summed over the wave; every lane then stages the chunk slots lane,
-// lane + 64, ... (range found by binary search over the prefix sums), so the
-// record loads of a chunk are independent and issued together.  Ranges that
-// may hold the target itself (self ranges) form a second stream of their own
-// that stages the particle ids too and runs a masked evaluation skipping the
-// self pair; the other ranges run the branch-free loop.
+// The item's source list (records of the ranges that cannot hold one of its
+// targets, then those of the self ranges) is staged into LDS in chunks of NCH
+// records: every lane stages the chunk slots lane, lane + 64, ..., so the
+// record loads of a chunk are independent and issued together.  Self sources
+// stage their particle ids too and run a masked evaluation that skips the
+// self pair; the others run the branch-free loop.
 template <typename Real, int R, bool SPH>
 __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
-                                             const Range *__restrict__ ranges,
+                                             const int32_t *__restrict__ near_src,
                                              const typename Rec<Real>::T4 *__restrict__ src,
                                              const int32_t *__restrict__ recid,
                                              const int32_t *__restrict__ recnode, int64_t p0, int p,
@@ -471,9 +506,11 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
                                                                                  : sizeof(double) * 64 * 3 * R;
     __shared__ __align__(32) unsigned char smem[LDS_BYTES];
     __shared__ int32_t lid[NCH];
-    __shared__ int32_t r_pre[65], r_start[64];
     T4 *lds = reinterpret_cast<T4 *>(smem);
-    const NearItem it = items[blockIdx.x];
+    // Blocks go round-robin to the 8 XCDs: give every XCD a contiguous run of
+    // items (neighbouring leaves, largely the same sources) for its own L2.
+    const int nb = gridDim.x, q8 = nb >> 3, r8 = nb & 7, xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
+    const NearItem it = items[(xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + loc];
     const int lane = threadIdx.x;
     const int cnt = it.tgt_cnt;
     const int T = (cnt + R - 1) / R;
@@ -508,57 +545,28 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
 #pragma unroll
         for (int q = 0; q < R; q++) {
             Real *pq = A.part(q);
-            if (sid != tid[q]) near_pair<SPH>(xt[q], s, pq[0], pq[1], pq[2]);
+            near_pair_masked<SPH>(xt[q], s, sid == tid[q], pq[0], pq[1], pq[2]);
         }
     };
-    for (int rb = it.a_beg; rb < it.a_end; rb += 64) {
-      const int nr = min(64, it.a_end - rb);
-      Range rg;
-      rg.start = rg.count = rg.node = rg.pad = 0;
-      if (lane < nr) rg = ranges[rb + lane];
-      const bool rself = (rg.count & FSS_SELF_BIT) != 0;
-      const bool any_self = __any(rself);
-      for (int pass = 0; pass < (any_self ? 2 : 1); pass++) {
-        // pass 0: ranges without the self pair; pass 1: self ranges, masked
-        const int c = (rself == (pass == 1)) ? (rg.count & ~FSS_SELF_BIT) : 0;
-        // inclusive scan of the range lengths over the wave
-        int incl = c;
+    const int32_t *list = near_src + it.src_off;
+    const int n0 = it.n_other, total = n0 + it.n_self;
+    for (int base = 0; base < total; base += NCH) {
+        const int m = min(NCH, total - base);
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            int v = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += v;
-        }
-        const int total = __shfl(incl, 63, 64);
-        __syncthreads(); // the previous batch is done with the range tables
-        r_pre[lane] = incl - c;
-        r_start[lane] = rg.start;
-        if (lane == 0) r_pre[64] = total;
-        __syncthreads();
-        for (int base = 0; base < total; base += NCH) {
-            const int m = min(NCH, total - base);
-#pragma unroll
-            for (int s = 0; s < NCH / 64; s++) {
-                const int k = lane + 64 * s;
-                if (k < m) {
-                    const int pos = base + k;
-                    // last range with r_pre <= pos, branch-free (entries from nr
-                    // on hold the total, which exceeds pos)
-                    int lo = 0;
-#pragma unroll
-                    for (int step = 32; step > 0; step >>= 1)
-                        if (r_pre[lo + step] <= pos) lo += step;
-                    const int64_t j = (int64_t)r_start[lo] + (pos - r_pre[lo]);
-                    lds[k] = to_frame(src[j], src_node(j, p0, p, recnode), centres, ct);
-                    if (pass == 1) lid[k] = recid[j];
-                }
+        for (int s = 0; s < NCH / 64; s++) {
+            const int k = lane + 64 * s;
+            if (k < m) {
+                const int32_t j = list[base + k];
+                lds[k] = to_frame(src[j], src_node(j, p0, p, recnode), centres, ct);
+                if (base + k >= n0) lid[k] = recid[j];
             }
-            __syncthreads();
-            if (pass == 1) run_chunk(m, ph, P, act, eval_masked);
-            else run_chunk(m, ph, P, act, eval);
-            A.flush();
-            __syncthreads();
         }
-      }
+        __syncthreads();
+        const int mu = min(m, max(0, n0 - base)); // slots [0, mu) other, [mu, m) self sources
+        run_chunk(mu, ph, P, act, eval);
+        if (mu < m) run_chunk_from(mu, m, ph, P, act, eval_masked);
+        A.flush();
+        __syncthreads();
     }
     double *red = reinterpret_cast<double *>(smem); // [64][3R], after the last chunk
 #pragma unroll
@@ -998,7 +1006,7 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
             auto kn = bltc ? (ctx->near_r == 2 ? k_near<double, 2, true> : k_near<double, 4, true>)
                            : (ctx->near_r == 2 ? k_near<double, 2, false> : k_near<double, 4, false>);
             hipLaunchKernelGGL(kn, dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
-                               T.ranges_a.as<Range>(), src, T.recid.as<int32_t>(), T.recnode.as<int32_t>(), p0, p,
+                               T.near_src.as<int32_t>(), src, T.recid.as<int32_t>(), T.recnode.as<int32_t>(), p0, p,
                                T.tri_info.as<double>(), pp->x, pp->y, pp->z, ost, ctx->out3.as<double>());
         }
         if (ev) FSS_HIP(hipEventRecord(ev[2], s));
@@ -1037,7 +1045,7 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
         if (T.n_near > 0) {
             auto kn = ctx->near_r == 4 ? k_near<float, 4, false> : k_near<float, 2, false>; // default 2
             hipLaunchKernelGGL(kn, dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
-                               T.ranges_a.as<Range>(), src, T.recid.as<int32_t>(), T.recnode.as<int32_t>(), p0, p,
+                               T.near_src.as<int32_t>(), src, T.recid.as<int32_t>(), T.recnode.as<int32_t>(), p0, p,
                                T.tri_info.as<double>(), pp->x, pp->y, pp->z, ost, ctx->out3.as<double>());
         }
         if (ev) FSS_HIP(hipEventRecord(ev[2], s));

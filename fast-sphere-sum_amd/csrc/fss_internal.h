@@ -84,6 +84,9 @@ struct NearItem {
     int32_t a_end;
     int32_t node;      // global index of the target node
     int32_t out_base;  // output slot of the first target
+    int64_t src_off;   // this item's source list in Tree::near_src (set by plan_finish)
+    int32_t n_other;   // sources of ranges that cannot hold a target, listed first
+    int32_t n_self;    // sources of self ranges, listed after them
 };
 
 // A source range: records [start, start + count), all members (or the
@@ -182,6 +185,8 @@ struct Tree {
     // evaluation plan
     int64_t n_near = 0, n_proxy = 0, n_slot = 0, n_ra = 0, n_rb = 0, n_charge = 0;
     DevBuf near_items;  // NearItem[n_near]
+    DevBuf near_src;    // int32 source record positions of every near item, item by item
+    int64_t n_near_src = 0;
     DevBuf proxy_items; // ProxyItem[n_proxy]
     DevBuf proxy_slots; // ProxySlot[n_slot]
     // proxy items by kernel: plist[0, n_plane) one lane per proxy (proxy and

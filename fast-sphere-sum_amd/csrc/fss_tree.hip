@@ -681,7 +681,7 @@ void tree_release(Tree &t)
     DevBuf *bufs[] = {&t.tri_xyz, &t.tri_info, &t.loc, &t.found, &t.counts, &t.offsets, &t.ids,
                       &t.inter, &t.inv_active, &t.nodeof, &t.node_slot, &t.proxy_slots, &t.recid,
                       &t.recnode, &t.nrange,
-                      &t.near_items, &t.proxy_items, &t.ranges_a,
+                      &t.near_items, &t.near_src, &t.proxy_items, &t.ranges_a,
                       &t.ranges_b, &t.charge_nodes, &t.proxy_xyz, &t.up_items, &t.up_multi, &t.tinv,
                       &t.rinfo, &t.oslot, &t.plist, &t.ipos};
     for (DevBuf *b : bufs) b->release();
@@ -1106,6 +1106,48 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
     return FSS_OK;
 }
 
+// Source list of every near item: the records of its class-A ranges, the
+// ranges that cannot hold one of the item's targets first, then the self
+// ranges.  One wave per item.
+__global__ void __launch_bounds__(64) k_near_src_count(NearItem *items, const Range *__restrict__ ranges,
+                                                       int64_t *cnt)
+{
+    const int lane = threadIdx.x;
+    NearItem it = items[blockIdx.x];
+    int c0 = 0, c1 = 0;
+    for (int r = it.a_beg + lane; r < it.a_end; r += 64) {
+        const int32_t c = ranges[r].count;
+        if (c & FSS_SELF_BIT) c1 += c & ~FSS_SELF_BIT;
+        else c0 += c;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        c0 += __shfl_xor(c0, off, 64);
+        c1 += __shfl_xor(c1, off, 64);
+    }
+    if (lane == 0) {
+        items[blockIdx.x].n_other = c0;
+        items[blockIdx.x].n_self = c1;
+        cnt[blockIdx.x] = (int64_t)c0 + c1;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_near_src_fill(NearItem *items, const Range *__restrict__ ranges,
+                                                      const int64_t *__restrict__ off, int32_t *near_src)
+{
+    const int lane = threadIdx.x;
+    const NearItem it = items[blockIdx.x];
+    int64_t o = off[blockIdx.x];
+    if (lane == 0) items[blockIdx.x].src_off = o;
+    for (int pass = 0; pass < 2; pass++)
+        for (int r = it.a_beg; r < it.a_end; r++) {
+            const Range rg = ranges[r];
+            if (((rg.count & FSS_SELF_BIT) != 0) != (pass == 1)) continue;
+            const int c = rg.count & ~FSS_SELF_BIT;
+            for (int i = lane; i < c; i += 64) near_src[o + i] = rg.start + i;
+            o += c;
+        }
+}
+
 fss_status plan_finish(fss_ctx *ctx)
 {
     Tree &T = ctx->tree;
@@ -1114,6 +1156,29 @@ fss_status plan_finish(fss_ctx *ctx)
     hipLaunchKernelGGL(k_tinv, dim3((unsigned)div_up(T.nn, 256)), dim3(256), 0, s, T.tri_xyz.as<double>(), T.nn,
                        T.tinv.as<double>());
     FSS_HIP(hipGetLastError());
+    // near-item source lists
+    T.n_near_src = 0;
+    if (T.n_near > 0) {
+        DevBuf cnt, off, tmp;
+        FSS_TRY(cnt.ensure(sizeof(int64_t) * T.n_near));
+        FSS_TRY(off.ensure(sizeof(int64_t) * T.n_near));
+        hipLaunchKernelGGL(k_near_src_count, dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
+                           T.ranges_a.as<Range>(), cnt.as<int64_t>());
+        FSS_HIP(hipGetLastError());
+        size_t bytes = 0;
+        FSS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt.as<int64_t>(), off.as<int64_t>(), (int)T.n_near, s));
+        FSS_TRY(tmp.ensure(bytes));
+        FSS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, cnt.as<int64_t>(), off.as<int64_t>(), (int)T.n_near, s));
+        int64_t last[2] = {0, 0};
+        FSS_TRY(d2h(&last[0], off.as<int64_t>() + T.n_near - 1, sizeof(int64_t), s));
+        FSS_TRY(d2h(&last[1], cnt.as<int64_t>() + T.n_near - 1, sizeof(int64_t), s));
+        T.n_near_src = last[0] + last[1];
+        FSS_TRY(T.near_src.ensure(sizeof(int32_t) * std::max<int64_t>(1, T.n_near_src)));
+        hipLaunchKernelGGL(k_near_src_fill, dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
+                           T.ranges_a.as<Range>(), off.as<int64_t>(), T.near_src.as<int32_t>());
+        FSS_HIP(hipGetLastError());
+        FSS_HIP(hipStreamSynchronize(s));
+    }
     // upward items: each charge node's records (main range, then exception
     // range) in chunks of FSS_UP_CHUNK
     std::vector<int32_t> charge(T.n_charge);
