@@ -82,19 +82,6 @@ __device__ inline void pair_eval(const double4 &xt, const double4 &s, double, do
     S2 = fma(c, s.z - xt.z, S2);
 }
 
-// FP64 near field of the treecode (eval_mode BLTC): 1 - x.y = |x - y|^2 / 2 on
-// the unit sphere (particles and source proxies lie on it); S accumulates
-// w (x_j - x_i) / |x_j - x_i|^2 and the caller doubles it.  Three VALU fewer
-// than the reference form per pair and free of its cancellation.
-__device__ inline void pair_eval_sph(const double4 &xt, const double4 &s, double &S0, double &S1, double &S2)
-{
-    double d0 = s.x - xt.x, d1 = s.y - xt.y, d2 = s.z - xt.z;
-    double c = s.w * rcp64(fma(d0, d0, fma(d1, d1, d2 * d2)));
-    S0 = fma(c, d0, S0);
-    S1 = fma(c, d1, S1);
-    S2 = fma(c, d2, S2);
-}
-
 // FP32: records are offsets from a common node centre and carry the doubled
 // weight 2 w; 2 (1 - x.y) = |d|^2 + off2, where off2 = 1 - |y|^2 for a target
 // off the sphere (planar proxy), so c = 2 w / (|d|^2 + off2) = w / (1 - x.y).
@@ -107,20 +94,6 @@ __device__ inline void pair_eval(const float4 &xt, const float4 &s, float off2, 
     S0 = fmaf(c, dx, S0);
     S1 = fmaf(c, dy, S1);
     S2 = fmaf(c, dz, S2);
-}
-
-// near-field pair of k_near: FP32 records always use the offset form; FP64
-// records the reference form (SPH false) or the sphere form (SPH true)
-template <bool SPH>
-__device__ inline void near_pair(const double4 &xt, const double4 &s, double &S0, double &S1, double &S2)
-{
-    if (SPH) pair_eval_sph(xt, s, S0, S1, S2);
-    else pair_eval(xt, s, 0.0, S0, S1, S2);
-}
-template <bool SPH>
-__device__ inline void near_pair(const float4 &xt, const float4 &s, float &S0, float &S1, float &S2)
-{
-    pair_eval(xt, s, 0.f, S0, S1, S2);
 }
 
 // The same pair with the self pair masked out (ranges that hold the target
@@ -368,17 +341,12 @@ __global__ void __launch_bounds__(64) k_upward_finish(const int4 *__restrict__ m
 // staged source loop shared by the near and proxy kernels
 // ------------------------------------------------------------------
 // Evaluates the staged entries [0, fill) owned by this lane (k = ph, ph + P,
-// ...), two per iteration, without branches in the loop body.
+// ...) without branches in the loop body.
 template <typename Eval>
 __device__ inline void run_chunk(int fill, int ph, int P, bool act, Eval eval)
 {
     if (!act) return;
-    int k = ph;
-    for (; k + P < fill; k += 2 * P) {
-        eval(k);
-        eval(k + P);
-    }
-    if (k < fill) eval(k);
+    for (int k = ph; k < fill; k += P) eval(k);
 }
 
 // Evaluates the staged entries [lo, fill) owned by this lane (the same slot to
@@ -481,6 +449,61 @@ template <int R> struct Acc<float, R> {
     __device__ double total(int q, int c) const { return S[q][c]; }
 };
 
+// Near field of the treecode in FP64 (SPH, eval_mode BLTC): 1 - x.y is taken
+// as |x - y|^2 / 2, equal on the unit sphere (particles and source proxies
+// lie on it) and free of the cancellation of 1 - x.y for close pairs; S
+// accumulates w (x_j - x_i) / |x_j - x_i|^2 and k_near doubles it.  Without
+// SPH (the reference dispatch) the denominator is the reference's 1 - x.y.
+//
+// One source against the R targets of a lane.  FP64: one reciprocal for the R
+// denominators -- prefix products P_q = den_0 ... den_q, r = w / P_{R-1}, then
+// backward c_q = P_{q-1} r, r <- den_q r -- which saves R - 1 of the
+// quarter-rate v_rcp_f64 and their Newton steps (relative error of each c
+// within a few ulp more than one reciprocal per pair); a zero denominator
+// (coincident points) falls back to one reciprocal per pair.  FP32: one pair
+// at a time.
+template <int R, bool SPH, typename AccT>
+__device__ inline void near_tile(const double4 *xt, const double4 &s, AccT &A)
+{
+    double d[R][3], den[R], pre[R], c[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        d[q][0] = s.x - xt[q].x;
+        d[q][1] = s.y - xt[q].y;
+        d[q][2] = s.z - xt[q].z;
+        den[q] = SPH ? fma(d[q][0], d[q][0], fma(d[q][1], d[q][1], d[q][2] * d[q][2])) : 1.0 - dot_ref(xt[q], s);
+        pre[q] = q == 0 ? den[0] : pre[q - 1] * den[q];
+    }
+    if (pre[R - 1] != 0.0) {
+        double rw = s.w * rcp64(pre[R - 1]);
+#pragma unroll
+        for (int q = R - 1; q > 0; q--) {
+            c[q] = pre[q - 1] * rw;
+            rw = den[q] * rw;
+        }
+        c[0] = rw;
+    } else {
+#pragma unroll
+        for (int q = 0; q < R; q++) c[q] = s.w * rcp64(den[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        double *pq = A.part(q);
+        pq[0] = fma(c[q], d[q][0], pq[0]);
+        pq[1] = fma(c[q], d[q][1], pq[1]);
+        pq[2] = fma(c[q], d[q][2], pq[2]);
+    }
+}
+template <int R, bool SPH, typename AccT>
+__device__ inline void near_tile(const float4 *xt, const float4 &s, AccT &A)
+{
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        float *pq = A.part(q);
+        pair_eval(xt[q], s, 0.f, pq[0], pq[1], pq[2]);
+    }
+}
+
 // ------------------------------------------------------------------
 // near field (class A): direct pairs, R targets per lane
 // ------------------------------------------------------------------
@@ -531,14 +554,7 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
     }
     Acc<Real, R> A;
     A.zero();
-    auto eval = [&](int k) {
-        T4 s = lds[k];
-#pragma unroll
-        for (int q = 0; q < R; q++) {
-            Real *pq = A.part(q);
-            near_pair<SPH>(xt[q], s, pq[0], pq[1], pq[2]);
-        }
-    };
+    auto eval = [&](int k) { near_tile<R, SPH>(xt, lds[k], A); };
     auto eval_masked = [&](int k) {
         T4 s = lds[k];
         int32_t sid = lid[k];
