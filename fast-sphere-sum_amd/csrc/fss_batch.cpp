@@ -54,8 +54,6 @@ struct Part {
     std::vector<ProxyItem> prox;
     std::vector<ProxySlot> slots;
     std::vector<Range> ra, rb;
-    std::vector<int32_t> plist;
-    int64_t n_plane;
     fss_tree_stats stats;
 };
 
@@ -103,8 +101,6 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
         FSS_TRY(down(P.slots, T.proxy_slots, T.n_slot, s));
         FSS_TRY(down(P.ra, T.ranges_a, T.n_ra, s));
         FSS_TRY(down(P.rb, T.ranges_b, T.n_rb, s));
-        FSS_TRY(down(P.plist, T.plist, T.n_proxy, s));
-        P.n_plane = T.n_plane;
         pb += ns[k];
     }
     // offsets
@@ -157,7 +153,6 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
     std::vector<ProxySlot> slots;
     std::vector<Range> ra, rb;
     std::vector<Instance> inst(ninst);
-    std::vector<int32_t> plane, pstaged;
     fss_tree_stats tot{};
     for (int k = 0; k < ninst; k++) {
         Part &P = parts[k];
@@ -200,8 +195,6 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
             sl.part_end = (int32_t)(sl.part_end + ptbase[k]);
             slots.push_back(sl);
         }
-        for (int64_t q = 0; q < (int64_t)P.plist.size(); q++)
-            (q < P.n_plane ? plane : pstaged).push_back((int32_t)(P.plist[q] + ptbase[k]));
         for (const Range &r : P.ra) ra.push_back(remap_range(r, k));
         for (const Range &r : P.rb) rb.push_back(remap_range(r, k));
         inst[k] = Instance{pbase[k], P.n, lbase[k], P.L, 0, sbase[k], nbase[k]};
@@ -224,8 +217,6 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
     T.ninter = tot.n_interactions;
     T.n_near = near.size();
     T.n_proxy = prox.size();
-    T.n_plane = plane.size();
-    plane.insert(plane.end(), pstaged.begin(), pstaged.end());
     T.n_slot = slots.size();
     T.n_ra = ra.size();
     T.n_rb = rb.size();
@@ -255,7 +246,6 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
     FSS_TRY(up(T.near_items, near, s));
     FSS_TRY(up(T.proxy_items, prox, s));
     FSS_TRY(up(T.proxy_slots, slots, s));
-    FSS_TRY(up(T.plist, plane, s));
     FSS_TRY(up(T.ranges_a, ra, s));
     FSS_TRY(up(T.ranges_b, rb, s));
     FSS_TRY(up(T.inst, inst, s));

@@ -253,27 +253,26 @@ __device__ inline void bary_inv(const double *__restrict__ R, const double *p, d
     for (int m = 0; m < 3; m++) c[m] = fma(R[3 * m], p[0], fma(R[3 * m + 1], p[1], R[3 * m + 2] * p[2]));
 }
 
-// modified charges q_k = sum_idx Vinv[idx][k] M_idx from the moments M
+// modified charge q_k = sum_idx Vinv[idx][k] M_idx from the moments M, for
+// one k (one lane per charge)
 template <typename Real, int D>
-__device__ inline void write_charges(const double *M, const double *__restrict__ vinv, int64_t p0, int32_t g,
-                                     typename Rec<Real>::T4 *src)
+__device__ inline void write_charge(const double *M, const double *__restrict__ vinv, int64_t p0, int32_t g,
+                                    int k, typename Rec<Real>::T4 *src)
 {
     constexpr int P = npts<D>();
-    for (int k = 0; k < P; k++) {
-        double q = 0.0;
+    double q = 0.0;
 #pragma unroll
-        for (int idx = 0; idx < P; idx++) q = fma(vinv[idx + k * P], M[idx], q); // vinv column-major
-        // weight slot of the proxy record (FP32 records carry 2 q)
-        src[p0 + (int64_t)g * P + k].w = (Real)(sizeof(Real) == 4 ? 2.0 * q : q);
-    }
+    for (int idx = 0; idx < P; idx++) q = fma(vinv[idx + k * P], M[idx], q); // vinv column-major
+    // weight slot of the proxy record (FP32 records carry 2 q)
+    src[p0 + (int64_t)g * P + k].w = (Real)(sizeof(Real) == 4 ? 2.0 * q : q);
 }
 
 // one wave per upward item (<= FSS_UP_CHUNK consecutive records of one
 // charge node, read coalesced): moments M_idx = sum_j m_idx(b_j) w_j with
 // normalised barycentric b_j; an item that covers its whole node writes the
-// charges, the others write partial moments
+// charges (lane k < P charge k), the others write partial moments
 template <typename Real, int D>
-__global__ void __launch_bounds__(64) k_upward(const int4 *__restrict__ items,
+__global__ void __launch_bounds__(256) k_upward(const int4 *__restrict__ items, int64_t nitems,
                                                const int32_t *__restrict__ recnode,
                                                const double *__restrict__ centres,
                                                const double *__restrict__ tinv,
@@ -281,8 +280,12 @@ __global__ void __launch_bounds__(64) k_upward(const int4 *__restrict__ items,
                                                typename Rec<Real>::T4 *src, double *partial)
 {
     constexpr int P = npts<D>();
-    const int4 it = items[blockIdx.x];
-    const int lane = threadIdx.x;
+    // four independent waves per block, one item each (more resident waves
+    // than one-wave blocks allow)
+    const int64_t wi = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wi >= nitems) return;
+    const int4 it = items[wi];
+    const int lane = threadIdx.x & 63;
     const double *R = tinv + 9 * (int64_t)it.x;
     double Rl[9];
 #pragma unroll
@@ -295,7 +298,7 @@ __global__ void __launch_bounds__(64) k_upward(const int4 *__restrict__ items,
         double pt[3], c[3], m[P], w;
         rec_point(src[pos], recnode[pos], centres, pt, w);
         bary_inv(Rl, pt, c);
-        double is = 1.0 / (c[0] + c[1] + c[2]);
+        double is = rcp64(c[0] + c[1] + c[2]);
         monomials<D>(c[0] * is, c[1] * is, m);
 #pragma unroll
         for (int k = 0; k < P; k++) M[k] = fma(m[k], w, M[k]);
@@ -303,26 +306,31 @@ __global__ void __launch_bounds__(64) k_upward(const int4 *__restrict__ items,
 #pragma unroll
     for (int k = 0; k < P; k++)
         for (int off = 32; off > 0; off >>= 1) M[k] += __shfl_xor(M[k], off, 64);
-    if (lane != 0) return;
+    // every lane holds M: lane k < P writes charge k (or partial moment k)
+    if (lane >= P) return;
     if (it.w < 0) {
-        write_charges<Real, D>(M, vinv, p0, it.x, src);
+        write_charge<Real, D>(M, vinv, p0, it.x, lane, src);
     } else {
-        double *o = partial + (int64_t)it.w * P;
+        double mk = M[0];
 #pragma unroll
-        for (int k = 0; k < P; k++) o[k] = M[k];
+        for (int k = 1; k < P; k++) mk = lane == k ? M[k] : mk;
+        partial[(int64_t)it.w * P + lane] = mk;
     }
 }
 
-// nodes with several items: one wave per node sums the partial moments
+// nodes with several items: one wave per node sums the partial moments (in
+// a fixed order) and lanes k < P write q_k
 template <typename Real, int D>
-__global__ void __launch_bounds__(64) k_upward_finish(const int4 *__restrict__ multi,
-                                                      const double *__restrict__ partial,
-                                                      const double *__restrict__ vinv, int64_t p0,
-                                                      typename Rec<Real>::T4 *src)
+__global__ void __launch_bounds__(256) k_upward_finish(const int4 *__restrict__ multi, int64_t nmulti,
+                                                       const double *__restrict__ partial,
+                                                       const double *__restrict__ vinv, int64_t p0,
+                                                       typename Rec<Real>::T4 *src)
 {
     constexpr int P = npts<D>();
-    const int4 mn = multi[blockIdx.x];
-    const int lane = threadIdx.x;
+    const int64_t wi = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wi >= nmulti) return;
+    const int4 mn = multi[wi];
+    const int lane = threadIdx.x & 63;
     double M[P];
 #pragma unroll
     for (int k = 0; k < P; k++) M[k] = 0.0;
@@ -334,7 +342,7 @@ __global__ void __launch_bounds__(64) k_upward_finish(const int4 *__restrict__ m
 #pragma unroll
     for (int k = 0; k < P; k++)
         for (int off = 32; off > 0; off >>= 1) M[k] += __shfl_xor(M[k], off, 64);
-    if (lane == 0) write_charges<Real, D>(M, vinv, p0, mn.x, src);
+    if (lane < P) write_charge<Real, D>(M, vinv, p0, mn.x, lane, src);
 }
 
 // ------------------------------------------------------------------
@@ -356,39 +364,6 @@ __device__ inline void run_chunk_from(int lo, int fill, int ph, int P, bool act,
 {
     if (!act) return;
     for (int k = ph < lo ? ph + (lo - ph + P - 1) / P * P : ph; k < fill; k += P) eval(k);
-}
-
-// Stages the sources of ranges [r0, r1) that are not flagged self into LDS in
-// chunks of NCH; after every full chunk the lane's entries are evaluated and
-// flush() moves the FP32 partial sums into FP64.  Records are moved to the
-// target node's frame (to_frame) while staged.  The caller evaluates the last
-// (partial) chunk.
-template <typename T4, typename Eval, typename Flush>
-__device__ inline void staged_ranges(const Range *__restrict__ ranges, int r0, int r1,
-                                     const T4 *__restrict__ src, const int32_t *__restrict__ recnode,
-                                     int64_t p0, int p, const double *__restrict__ centres, const double *ct,
-                                     T4 *lds, int ph, int P, bool act, Eval eval, Flush flush, int &fill)
-{
-    const int lane = threadIdx.x;
-    for (int r = r0; r < r1; r++) {
-        Range rg = ranges[r];
-        if (rg.count & FSS_SELF_BIT) continue;
-        for (int base = 0; base < rg.count; base += 64) {
-            int m = min(64, rg.count - base);
-            if (fill + m > NCH) {
-                __syncthreads();
-                run_chunk(fill, ph, P, act, eval);
-                flush();
-                __syncthreads();
-                fill = 0;
-            }
-            if (lane < m) {
-                int64_t j = (int64_t)rg.start + base + lane;
-                lds[fill + lane] = to_frame(src[j], src_node(j, p0, p, recnode), centres, ct);
-            }
-            fill += m;
-        }
-    }
 }
 
 // target position as a double (absolute for FP64 records, from the input
@@ -621,109 +596,87 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
 // ------------------------------------------------------------------
 // proxy values (class B): p proxies of a node against one chunk of sources
 // ------------------------------------------------------------------
-// One lane per (item, proxy): a wave holds 64 / p items and every lane walks
-// its item's source ranges itself (the p lanes of an item read the same
-// source records, a broadcast).  No cross-lane reduction.  FP32 proxy source
-// ranges (cc) move to the target frame with one shift per range; particle
-// ranges per record.
-template <typename Real, int D>
-__global__ void __launch_bounds__(64) k_proxy(const ProxyItem *__restrict__ items,
-                                              const int32_t *__restrict__ list, int64_t nlist,
-                                              const Range *__restrict__ ranges,
-                                              const typename Rec<Real>::T4 *__restrict__ src,
-                                              const int32_t *__restrict__ recnode,
-                                              const float *__restrict__ aoff, int64_t p0,
-                                              const double *__restrict__ centres,
-                                              const double4 *__restrict__ proxy_xyz, double *phi)
+// One source against the R proxies of a lane.  FP64: one reciprocal for the R
+// denominators as in near_tile (far pairs are separated by the acceptance
+// criterion, so no denominator vanishes).  FP32: one pair at a time.
+template <int R, typename AccT>
+__device__ inline void far_tile(const double4 *yk, const double4 &s, const double *, AccT &A)
 {
-    using T4 = typename Rec<Real>::T4;
-    constexpr int PP = npts<D>();
-    constexpr int IPW = 64 / PP;
-    const int lane = threadIdx.x;
-    const int il = lane / PP, k = lane - il * PP;
-    const int64_t li = (int64_t)blockIdx.x * IPW + il;
-    if (il >= IPW || li >= nlist) return;
-    const int64_t item = list[li];
-    const ProxyItem it = items[item];
-    const int64_t pk = p0 + (int64_t)it.node * PP + k;
-    const T4 yk = src[pk];
-    const Real at = aoff ? (Real)aoff[pk - p0] : (Real)0;
-    const double *cp = centres + 4 * (int64_t)it.node;
-    const double ct[3] = {cp[0], cp[1], cp[2]};
-    Acc<Real, 1> A;
-    A.zero();
-    Real *pq = A.part(0);
-    Range nx;
-    nx.start = nx.count = nx.node = nx.pad = 0;
-    if (it.b_beg < it.b_end) nx = ranges[it.b_beg];
-    for (int r = it.b_beg; r < it.b_end; r++) {
-        const Range rg = nx;
-        if (r + 1 < it.b_end) nx = ranges[r + 1]; // next range descriptor in flight
-        const int cnt = rg.count & ~FSS_SELF_BIT;
-        if (rg.start >= p0) {
-            // the p proxies of one source node (count == p): loads issued in
-            // batches of up to 8, one frame shift for the range
-            T4 sh;
-            sh.x = sh.y = sh.z = sh.w = 0;
-            sh = to_frame(sh, rg.node, centres, ct);
-            constexpr int B = PP < 8 ? PP : 8;
+    double den[R], pre[R], c[R];
 #pragma unroll
-            for (int j0 = 0; j0 < PP; j0 += B) {
-                T4 s[B];
-#pragma unroll
-                for (int j = 0; j < B; j++)
-                    if (j0 + j < PP) s[j] = src[rg.start + j0 + j];
-#pragma unroll
-                for (int j = 0; j < B; j++) {
-                    if (j0 + j >= PP) continue;
-                    if constexpr (sizeof(Real) == 4) {
-                        s[j].x += sh.x;
-                        s[j].y += sh.y;
-                        s[j].z += sh.z;
-                    }
-                    pair_far(yk, s[j], at, pq[0], pq[1], pq[2]);
-                }
-            }
-            A.flush();
-        } else {
-            for (int j = 0; j < cnt; j++) {
-                const int64_t js = (int64_t)rg.start + j;
-                pair_far(yk, to_frame(src[js], recnode[js], centres, ct), at, pq[0], pq[1], pq[2]);
-                if ((j & 15) == 15) A.flush(); // FP32 partial sums of <= 16 terms
-            }
-            A.flush();
-        }
+    for (int q = 0; q < R; q++) {
+        den[q] = 1.0 - (yk[q].x * s.x + yk[q].y * s.y + yk[q].z * s.z);
+        pre[q] = q == 0 ? den[0] : pre[q - 1] * den[q];
     }
-    // y x x_j = y x (x_j - y): absolute FP64 proxy position
-    const double4 ya = proxy_xyz[(int64_t)it.node * PP + k];
-    const double a0 = A.total(0, 0), a1 = A.total(0, 1), a2 = A.total(0, 2);
-    double *o = phi + item * 3 * PP;
-    o[k] = ya.y * a2 - ya.z * a1;
-    o[PP + k] = ya.z * a0 - ya.x * a2;
-    o[2 * PP + k] = ya.x * a1 - ya.y * a0;
+    double rw = s.w * rcp64(pre[R - 1]);
+#pragma unroll
+    for (int q = R - 1; q > 0; q--) {
+        c[q] = pre[q - 1] * rw;
+        rw = den[q] * rw;
+    }
+    c[0] = rw;
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        double *pq = A.part(q);
+        pq[0] = fma(c[q], s.x, pq[0]);
+        pq[1] = fma(c[q], s.y, pq[1]);
+        pq[2] = fma(c[q], s.z, pq[2]);
+    }
+}
+template <int R, typename AccT>
+__device__ inline void far_tile(const float4 *yk, const float4 &s, const float *at, AccT &A)
+{
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        float *pq = A.part(q);
+        pair_far(yk[q], s, at[q], pq[0], pq[1], pq[2]);
+    }
 }
 
-// Items with long particle source ranges (reference dispatch: cp and cc -> cp
-// take every particle of the source node): sources staged in LDS in chunks,
-// lanes split into proxy groups x source phases, reduction over phases.
+// Orders one wave's LDS accesses without a workgroup barrier: the LDS
+// executes the instructions of a wave in order, so a compiler fence at
+// wavefront scope suffices between a lane's store and another lane's load.
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Proxy values from the item's source list: one wave per item (four
+// independent waves per block, each with its own LDS slice), the list staged
+// into LDS in chunks of NCH_PROXY records (all loads of a chunk independent),
+// lanes split into proxy groups x source phases, reduction over the phases
+// through LDS.
+constexpr int NCH_PROXY = 128;
 template <typename Real, int D>
-__global__ void __launch_bounds__(64) k_proxy_staged(const ProxyItem *__restrict__ items,
-                                                     const int32_t *__restrict__ list,
-                                              const Range *__restrict__ ranges,
-                                              const typename Rec<Real>::T4 *__restrict__ src,
-                                              const int32_t *__restrict__ recnode,
-                                              const float *__restrict__ aoff, int64_t p0,
-                                              const double *__restrict__ centres,
-                                              const double4 *__restrict__ proxy_xyz, double *phi)
+__global__ void __launch_bounds__(256) k_proxy_list(const ProxyItem *__restrict__ items, int64_t nitems,
+                                                   const int64_t *__restrict__ far_off,
+                                                   const int32_t *__restrict__ far_src,
+                                                   const typename Rec<Real>::T4 *__restrict__ src,
+                                                   const int32_t *__restrict__ recnode,
+                                                   const float *__restrict__ aoff, int64_t p0,
+                                                   const double *__restrict__ centres,
+                                                   const double4 *__restrict__ proxy_xyz, double *phi)
 {
     using T4 = typename Rec<Real>::T4;
     constexpr int PP = npts<D>();
-    constexpr int R = PP > 6 ? 2 : 1;
-    __shared__ T4 lds[NCH];
-    __shared__ double red[64][3 * R];
-    const int32_t item = list[blockIdx.x];
+    // proxies per lane: 3 up to degree 2, 2 beyond (fewer registers)
+    constexpr int R = PP <= 6 ? 3 : 2;
+    constexpr int TT = (PP + R - 1) / R;  // lanes per phase group
+    constexpr int NOUT = TT * 3 * R;       // phase sums: (group lane, proxy, component)
+    constexpr int RED_BYTES = sizeof(double) * (64 * 3 * R + NOUT);
+    constexpr int LDS_BYTES = sizeof(T4) * NCH_PROXY > RED_BYTES ? sizeof(T4) * NCH_PROXY : RED_BYTES;
+    __shared__ __align__(32) unsigned char smem_all[4][LDS_BYTES];
+    const int wave = threadIdx.x >> 6;
+    unsigned char *smem = smem_all[wave];
+    T4 *lds = reinterpret_cast<T4 *>(smem);
+    // contiguous runs of blocks per XCD (blocks go round-robin to the 8 XCDs)
+    const int nb = gridDim.x, q8 = nb >> 3, r8 = nb & 7, xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
+    const int64_t item = 4 * (int64_t)((xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + loc) + wave;
+    if (item >= nitems) return; // the whole wave: no block-wide barrier follows
     const ProxyItem it = items[item];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int T = (PP + R - 1) / R;
     const int P = 64 / T;
     const int ph = lane / T, tl = lane - ph * T;
@@ -738,49 +691,55 @@ __global__ void __launch_bounds__(64) k_proxy_staged(const ProxyItem *__restrict
         yk[q] = src[pk];
         at[q] = aoff ? (Real)aoff[pk - p0] : (Real)0;
     }
-    Acc<Real, R> A;
-    A.zero();
-    auto eval = [&](int k) {
-        T4 s = lds[k];
-#pragma unroll
-        for (int q = 0; q < R; q++) {
-            Real *pq = A.part(q);
-            pair_far(yk[q], s, at[q], pq[0], pq[1], pq[2]);
-        }
-    };
-    auto flush = [&]() { A.flush(); };
     const double *cp = centres + 4 * (int64_t)it.node;
     const double ct[3] = {cp[0], cp[1], cp[2]};
-    int fill = 0;
-    staged_ranges(ranges, it.b_beg, it.b_end, src, recnode, p0, PP, centres, ct, lds, ph, P, act, eval, flush,
-                  fill);
-    __syncthreads();
-    run_chunk(fill, ph, P, act, eval);
-    A.flush();
+    Acc<Real, R> A;
+    A.zero();
+    auto eval = [&](int k) { far_tile<R>(yk, lds[k], at, A); };
+    const int64_t o0 = far_off[item];
+    const int total = (int)(far_off[item + 1] - o0);
+    const int32_t *list = far_src + o0;
+    for (int base = 0; base < total; base += NCH_PROXY) {
+        const int m = min(NCH_PROXY, total - base);
+#pragma unroll
+        for (int s = 0; s < NCH_PROXY / 64; s++) {
+            const int k = lane + 64 * s;
+            if (k < m) {
+                const int32_t j = list[base + k];
+                lds[k] = to_frame(src[j], src_node(j, p0, PP, recnode), centres, ct);
+            }
+        }
+        wave_sync();
+        run_chunk(m, ph, P, act, eval);
+        A.flush();
+        wave_sync();
+    }
+    // sum over the phases: every (group lane, proxy, component) by its own lane
+    double *red = reinterpret_cast<double *>(smem); // [64][3R], after the last chunk
+    double *sum = red + 64 * 3 * R;                 // [T][3R]
 #pragma unroll
     for (int q = 0; q < R; q++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) red[lane][3 * q + c] = A.total(q, c);
-    __syncthreads();
-    if (ph == 0) {
-        double *o = phi + (int64_t)item * 3 * PP;
-#pragma unroll
-        for (int q = 0; q < R; q++) {
-            int k = tl + q * T;
-            if (k >= PP) continue;
-            double a0 = 0, a1 = 0, a2 = 0;
-            for (int h = 0; h < P; h++) {
-                a0 += red[h * T + tl][3 * q];
-                a1 += red[h * T + tl][3 * q + 1];
-                a2 += red[h * T + tl][3 * q + 2];
-            }
-            // y x x_j = y x (x_j - y): absolute FP64 proxy position
-            double4 ya = proxy_xyz[(int64_t)it.node * PP + k];
-            double y0 = ya.x, y1 = ya.y, y2 = ya.z;
-            o[k] = y1 * a2 - y2 * a1;
-            o[PP + k] = y2 * a0 - y0 * a2;
-            o[2 * PP + k] = y0 * a1 - y1 * a0;
-        }
+        for (int c = 0; c < 3; c++) red[lane * 3 * R + 3 * q + c] = A.total(q, c);
+    wave_sync();
+    for (int o = lane; o < NOUT; o += 64) {
+        const int g = o / (3 * R), r = o - g * (3 * R);
+        double a = 0.0;
+        for (int h = 0; h < P; h++) a += red[(h * T + g) * 3 * R + r];
+        sum[o] = a;
+    }
+    wave_sync();
+    for (int o = lane; o < T * R; o += 64) {
+        const int g = o / R, q = o - g * R, k = g + q * T;
+        if (k >= PP) continue;
+        const double *sq = sum + g * 3 * R + 3 * q;
+        const double a0 = sq[0], a1 = sq[1], a2 = sq[2];
+        // y x x_j = y x (x_j - y): absolute FP64 proxy position
+        const double4 ya = proxy_xyz[(int64_t)it.node * PP + k];
+        double *out = phi + item * 3 * PP;
+        out[k] = ya.y * a2 - ya.z * a1;
+        out[PP + k] = ya.z * a0 - ya.x * a2;
+        out[2 * PP + k] = ya.x * a1 - ya.y * a0;
     }
 }
 
@@ -971,6 +930,27 @@ __global__ void k_scatter(const int32_t *__restrict__ ipos, int64_t n, const dou
 // ------------------------------------------------------------------
 // host launcher
 // ------------------------------------------------------------------
+// BLTC modified charges of every charge node into the proxy records of src
+template <typename Real, int D>
+static fss_status upward(fss_ctx *ctx, typename Rec<Real>::T4 *src)
+{
+    Tree &T = ctx->tree;
+    hipStream_t s = ctx->stream;
+    constexpr int p = npts<D>();
+    const int64_t p0 = T.p0;
+    FSS_TRY(ctx->upart.ensure(sizeof(double) * p * std::max<int64_t>(1, T.n_up_partial)));
+    double *part = ctx->upart.as<double>();
+    if (T.n_up == 0) return FSS_OK;
+    hipLaunchKernelGGL((k_upward<Real, D>), dim3((unsigned)div_up(T.n_up, 4)), dim3(256), 0, s,
+                       T.up_items.as<int4>(), T.n_up, T.recnode.as<int32_t>(), T.tri_info.as<double>(),
+                       T.tinv.as<double>(), ctx->d_vinv.as<double>(), p0, src, part);
+    if (T.n_up_multi > 0)
+        hipLaunchKernelGGL((k_upward_finish<Real, D>), dim3((unsigned)div_up(T.n_up_multi, 4)), dim3(256), 0, s,
+                           T.up_multi.as<int4>(), T.n_up_multi, part, ctx->d_vinv.as<double>(), p0, src);
+    FSS_HIP(hipGetLastError());
+    return FSS_OK;
+}
+
 template <int D>
 static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity *out, hipEvent_t *ev)
 {
@@ -985,7 +965,6 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
     FSS_TRY(ctx->out3.ensure(sizeof(double) * 3 * ost));
     FSS_TRY(ctx->phi.ensure(sizeof(double) * 3 * p * std::max<int64_t>(1, T.n_proxy)));
     FSS_TRY(ctx->alpha.ensure(sizeof(double) * 3 * p * std::max<int64_t>(1, T.n_slot)));
-    FSS_TRY(ctx->upart.ensure(sizeof(double) * p * std::max<int64_t>(1, T.n_up_partial)));
     if (ev) FSS_HIP(hipEventRecord(ev[0], s));
     unsigned gs = (unsigned)div_up(nsrc, 256);
     if (!mixed) {
@@ -993,28 +972,13 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
         double4 *src = ctx->src64.as<double4>();
         hipLaunchKernelGGL(k_gather64, dim3(gs), dim3(256), 0, s, pp->x, pp->y, pp->z, pp->vort,
                            pp->area, T.recid.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, src);
-        if (bltc && T.n_up > 0) {
-            hipLaunchKernelGGL((k_upward<double, D>), dim3((unsigned)T.n_up), dim3(64), 0, s,
-                               T.up_items.as<int4>(), T.recnode.as<int32_t>(), T.tri_info.as<double>(),
-                               T.tinv.as<double>(), ctx->d_vinv.as<double>(), p0, src,
-                               ctx->upart.as<double>());
-            if (T.n_up_multi > 0)
-                hipLaunchKernelGGL((k_upward_finish<double, D>), dim3((unsigned)T.n_up_multi), dim3(64), 0, s,
-                                   T.up_multi.as<int4>(), ctx->upart.as<double>(), ctx->d_vinv.as<double>(), p0,
-                                   src);
-        }
+        if (bltc) FSS_TRY((upward<double, D>(ctx, src)));
         if (T.n_proxy > 0)
         {
-            if (T.n_plane > 0)
-                hipLaunchKernelGGL((k_proxy<double, D>), dim3((unsigned)div_up(T.n_plane, 64 / p)), dim3(64), 0, s,
-                                   T.proxy_items.as<ProxyItem>(), T.plist.as<int32_t>(), T.n_plane,
-                                   T.ranges_b.as<Range>(), src, T.recnode.as<int32_t>(), (const float *)nullptr, p0,
-                                   T.tri_info.as<double>(), T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
-            if (T.n_proxy > T.n_plane)
-                hipLaunchKernelGGL((k_proxy_staged<double, D>), dim3((unsigned)(T.n_proxy - T.n_plane)), dim3(64), 0,
-                                   s, T.proxy_items.as<ProxyItem>(), T.plist.as<int32_t>() + T.n_plane,
-                                   T.ranges_b.as<Range>(), src, T.recnode.as<int32_t>(), (const float *)nullptr, p0,
-                                   T.tri_info.as<double>(), T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
+            hipLaunchKernelGGL((k_proxy_list<double, D>), dim3((unsigned)div_up(T.n_proxy, 4)), dim3(256), 0, s,
+                               T.proxy_items.as<ProxyItem>(), T.n_proxy, T.far_off.as<int64_t>(), T.far_src.as<int32_t>(), src,
+                               T.recnode.as<int32_t>(), (const float *)nullptr, p0, T.tri_info.as<double>(),
+                               T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
         }
         if (ev) FSS_HIP(hipEventRecord(ev[1], s));
         if (T.n_near > 0) {
@@ -1034,28 +998,13 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
                            pp->area, T.recid.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, p,
                            T.recnode.as<int32_t>(), T.tri_info.as<double>(), bltc ? 0 : 1, src,
                            aoff);
-        if (bltc && T.n_up > 0) {
-            hipLaunchKernelGGL((k_upward<float, D>), dim3((unsigned)T.n_up), dim3(64), 0, s,
-                               T.up_items.as<int4>(), T.recnode.as<int32_t>(), T.tri_info.as<double>(),
-                               T.tinv.as<double>(), ctx->d_vinv.as<double>(), p0, src,
-                               ctx->upart.as<double>());
-            if (T.n_up_multi > 0)
-                hipLaunchKernelGGL((k_upward_finish<float, D>), dim3((unsigned)T.n_up_multi), dim3(64), 0, s,
-                                   T.up_multi.as<int4>(), ctx->upart.as<double>(), ctx->d_vinv.as<double>(), p0,
-                                   src);
-        }
+        if (bltc) FSS_TRY((upward<float, D>(ctx, src)));
         if (T.n_proxy > 0)
         {
-            if (T.n_plane > 0)
-                hipLaunchKernelGGL((k_proxy<float, D>), dim3((unsigned)div_up(T.n_plane, 64 / p)), dim3(64), 0, s,
-                                   T.proxy_items.as<ProxyItem>(), T.plist.as<int32_t>(), T.n_plane,
-                                   T.ranges_b.as<Range>(), src, T.recnode.as<int32_t>(), (const float *)aoff, p0,
-                                   T.tri_info.as<double>(), T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
-            if (T.n_proxy > T.n_plane)
-                hipLaunchKernelGGL((k_proxy_staged<float, D>), dim3((unsigned)(T.n_proxy - T.n_plane)), dim3(64), 0,
-                                   s, T.proxy_items.as<ProxyItem>(), T.plist.as<int32_t>() + T.n_plane,
-                                   T.ranges_b.as<Range>(), src, T.recnode.as<int32_t>(), (const float *)aoff, p0,
-                                   T.tri_info.as<double>(), T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
+            hipLaunchKernelGGL((k_proxy_list<float, D>), dim3((unsigned)div_up(T.n_proxy, 4)), dim3(256), 0, s,
+                               T.proxy_items.as<ProxyItem>(), T.n_proxy, T.far_off.as<int64_t>(), T.far_src.as<int32_t>(), src,
+                               T.recnode.as<int32_t>(), (const float *)aoff, p0, T.tri_info.as<double>(),
+                               T.proxy_xyz.as<double4>(), ctx->phi.as<double>());
         }
         if (ev) FSS_HIP(hipEventRecord(ev[1], s));
         if (T.n_near > 0) {
@@ -1101,18 +1050,10 @@ template <int D> static fss_status sources64_degree(fss_ctx *ctx, const fss_part
     constexpr int p = npts<D>();
     const int64_t p0 = T.p0, nprox = T.nn * p, nsrc = p0 + nprox;
     FSS_TRY(ctx->src64.ensure(sizeof(double4) * nsrc));
-    FSS_TRY(ctx->upart.ensure(sizeof(double) * p * std::max<int64_t>(1, T.n_up_partial)));
     double4 *src = ctx->src64.as<double4>();
     hipLaunchKernelGGL(k_gather64, dim3((unsigned)div_up(nsrc, 256)), dim3(256), 0, s, pp->x, pp->y, pp->z, pp->vort,
                        pp->area, T.recid.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, src);
-    if (T.n_up > 0) {
-        hipLaunchKernelGGL((k_upward<double, D>), dim3((unsigned)T.n_up), dim3(64), 0, s, T.up_items.as<int4>(),
-                           T.recnode.as<int32_t>(), T.tri_info.as<double>(), T.tinv.as<double>(),
-                           ctx->d_vinv.as<double>(), p0, src, ctx->upart.as<double>());
-        if (T.n_up_multi > 0)
-            hipLaunchKernelGGL((k_upward_finish<double, D>), dim3((unsigned)T.n_up_multi), dim3(64), 0, s,
-                               T.up_multi.as<int4>(), ctx->upart.as<double>(), ctx->d_vinv.as<double>(), p0, src);
-    }
+    FSS_TRY((upward<double, D>(ctx, src)));
     FSS_HIP(hipGetLastError());
     return FSS_OK;
 }

@@ -123,8 +123,6 @@ struct ProxySlot {
 #define FSS_SELF_BIT (1 << 30)
 // Sources per proxy work item before a node's class-B list is split.
 #define FSS_PROXY_CHUNK 4096
-// Longest particle source range of a proxy item evaluated one lane per proxy.
-#define FSS_LANE_MAX 32
 // Records per (one-wave) item of the upward pass.
 #define FSS_UP_CHUNK 512
 
@@ -189,10 +187,8 @@ struct Tree {
     int64_t n_near_src = 0;
     DevBuf proxy_items; // ProxyItem[n_proxy]
     DevBuf proxy_slots; // ProxySlot[n_slot]
-    // proxy items by kernel: plist[0, n_plane) one lane per proxy (proxy and
-    // short particle ranges), plist[n_plane, n_proxy) LDS-staged (long ranges)
-    DevBuf plist;       // int32 [n_proxy]
-    int64_t n_plane = 0;
+    DevBuf far_off;     // int64 [n_proxy + 1]: source list of every proxy item in far_src
+    DevBuf far_src;     // int32 source record positions of every proxy item, item by item
     DevBuf ranges_a;    // Range [n_ra]
     DevBuf ranges_b;    // Range [n_rb]
     DevBuf inst;        // Instance[ninst]

@@ -681,9 +681,9 @@ void tree_release(Tree &t)
     DevBuf *bufs[] = {&t.tri_xyz, &t.tri_info, &t.loc, &t.found, &t.counts, &t.offsets, &t.ids,
                       &t.inter, &t.inv_active, &t.nodeof, &t.node_slot, &t.proxy_slots, &t.recid,
                       &t.recnode, &t.nrange,
-                      &t.near_items, &t.near_src, &t.proxy_items, &t.ranges_a,
+                      &t.near_items, &t.near_src, &t.proxy_items, &t.far_off, &t.far_src, &t.ranges_a,
                       &t.ranges_b, &t.charge_nodes, &t.proxy_xyz, &t.up_items, &t.up_multi, &t.tinv,
-                      &t.rinfo, &t.oslot, &t.plist, &t.ipos};
+                      &t.rinfo, &t.oslot, &t.ipos};
     for (DevBuf *b : bufs) b->release();
     t.built = false;
 }
@@ -1024,22 +1024,16 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
         if (T.n_rb > 0) FSS_TRY(d2h(rb.data(), T.ranges_b.p, sizeof(Range) * T.n_rb, s));
         std::vector<ProxyItem> parts;
         std::vector<ProxySlot> slots(T.n_slot);
-        std::vector<int32_t> lane, staged;
         for (int64_t q = 0; q < T.n_slot; q++) {
             const ProxyItem &it = one[q];
             slots[q].node = it.node;
             slots[q].part_beg = (int32_t)parts.size();
             slots[q].pad = 0;
-            // a node with a long particle range goes to the LDS-staged kernel
-            bool longr = false;
-            for (int32_t r = it.b_beg; r < it.b_end; r++)
-                longr = longr || (rb[r].start < T.p0 && (rb[r].count & ~FSS_SELF_BIT) > FSS_LANE_MAX);
             int32_t b = it.b_beg;
             int64_t acc = 0;
             for (int32_t r = it.b_beg; r < it.b_end; r++) {
                 acc += rb[r].count & ~FSS_SELF_BIT;
                 if (acc >= FSS_PROXY_CHUNK || r + 1 == it.b_end) {
-                    (longr ? staged : lane).push_back((int32_t)parts.size());
                     parts.push_back(ProxyItem{it.node, b, r + 1, (int32_t)q});
                     b = r + 1;
                     acc = 0;
@@ -1048,11 +1042,6 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
             slots[q].part_end = (int32_t)parts.size();
         }
         T.n_proxy = (int64_t)parts.size();
-        T.n_plane = (int64_t)lane.size();
-        lane.insert(lane.end(), staged.begin(), staged.end());
-        FSS_TRY(T.plist.ensure(sizeof(int32_t) * std::max<int64_t>(1, T.n_proxy)));
-        if (T.n_proxy > 0)
-            FSS_HIP(hipMemcpyAsync(T.plist.p, lane.data(), sizeof(int32_t) * T.n_proxy, hipMemcpyHostToDevice, s));
         FSS_TRY(T.proxy_items.ensure(sizeof(ProxyItem) * std::max<int64_t>(1, T.n_proxy)));
         FSS_TRY(T.proxy_slots.ensure(sizeof(ProxySlot) * std::max<int64_t>(1, T.n_slot)));
         if (T.n_proxy > 0)
@@ -1148,6 +1137,33 @@ __global__ void __launch_bounds__(64) k_near_src_fill(NearItem *items, const Ran
         }
 }
 
+// Source list of every proxy item: the records of its class-B ranges in order.
+__global__ void __launch_bounds__(64) k_far_src_count(const ProxyItem *__restrict__ items,
+                                                      const Range *__restrict__ ranges, int64_t *cnt)
+{
+    const int lane = threadIdx.x;
+    const ProxyItem it = items[blockIdx.x];
+    int c = 0;
+    for (int r = it.b_beg + lane; r < it.b_end; r += 64) c += ranges[r].count & ~FSS_SELF_BIT;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if (lane == 0) cnt[blockIdx.x] = c;
+}
+
+__global__ void __launch_bounds__(64) k_far_src_fill(const ProxyItem *__restrict__ items,
+                                                     const Range *__restrict__ ranges,
+                                                     const int64_t *__restrict__ off, int32_t *far_src)
+{
+    const int lane = threadIdx.x;
+    const ProxyItem it = items[blockIdx.x];
+    int64_t o = off[blockIdx.x];
+    for (int r = it.b_beg; r < it.b_end; r++) {
+        const Range rg = ranges[r];
+        const int c = rg.count & ~FSS_SELF_BIT;
+        for (int i = lane; i < c; i += 64) far_src[o + i] = rg.start + i;
+        o += c;
+    }
+}
+
 fss_status plan_finish(fss_ctx *ctx)
 {
     Tree &T = ctx->tree;
@@ -1176,6 +1192,30 @@ fss_status plan_finish(fss_ctx *ctx)
         FSS_TRY(T.near_src.ensure(sizeof(int32_t) * std::max<int64_t>(1, T.n_near_src)));
         hipLaunchKernelGGL(k_near_src_fill, dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
                            T.ranges_a.as<Range>(), off.as<int64_t>(), T.near_src.as<int32_t>());
+        FSS_HIP(hipGetLastError());
+        FSS_HIP(hipStreamSynchronize(s));
+    }
+    // proxy-item source lists; far_off[n_proxy] = total
+    if (T.n_proxy > 0) {
+        DevBuf cnt, tmp;
+        FSS_TRY(cnt.ensure(sizeof(int64_t) * (T.n_proxy + 1)));
+        FSS_TRY(T.far_off.ensure(sizeof(int64_t) * (T.n_proxy + 1)));
+        FSS_HIP(hipMemsetAsync(cnt.as<int64_t>() + T.n_proxy, 0, sizeof(int64_t), s));
+        hipLaunchKernelGGL(k_far_src_count, dim3((unsigned)T.n_proxy), dim3(64), 0, s,
+                           T.proxy_items.as<ProxyItem>(), T.ranges_b.as<Range>(), cnt.as<int64_t>());
+        FSS_HIP(hipGetLastError());
+        size_t bytes = 0;
+        FSS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt.as<int64_t>(), T.far_off.as<int64_t>(),
+                                                 (int)(T.n_proxy + 1), s));
+        FSS_TRY(tmp.ensure(bytes));
+        FSS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, cnt.as<int64_t>(), T.far_off.as<int64_t>(),
+                                                 (int)(T.n_proxy + 1), s));
+        int64_t total = 0;
+        FSS_TRY(d2h(&total, T.far_off.as<int64_t>() + T.n_proxy, sizeof(int64_t), s));
+        FSS_TRY(T.far_src.ensure(sizeof(int32_t) * std::max<int64_t>(1, total)));
+        hipLaunchKernelGGL(k_far_src_fill, dim3((unsigned)T.n_proxy), dim3(64), 0, s,
+                           T.proxy_items.as<ProxyItem>(), T.ranges_b.as<Range>(), T.far_off.as<int64_t>(),
+                           T.far_src.as<int32_t>());
         FSS_HIP(hipGetLastError());
         FSS_HIP(hipStreamSynchronize(s));
     }
