@@ -437,7 +437,7 @@ def main():
     alg_pairs = head["pairs_near"] - spairs
     flp = FLOPS_PAIR_FP64_SPH if hprec == fss.FP64 else FLOPS_PAIR_FP32
     peak = PEAK_FP64_TFLOPS if hprec == fss.FP64 else PEAK_FP32_TFLOPS
-    kname = "k_near<double,4>" if hprec == fss.FP64 else "k_near<float,2>"
+    kname = "k_near<double,4,true>" if hprec == fss.FP64 else "k_near<float,2,false>"
     achieved = alg_pairs * flp / (head["near_kernel_ms"] * 1e-3) / 1e12
     roofline = dict(kernel=kname, bound="valu", achieved=achieved, peak=peak,
                     unit="TFLOP/s", frac=achieved / peak,
