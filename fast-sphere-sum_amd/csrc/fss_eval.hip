@@ -699,17 +699,39 @@ __global__ void __launch_bounds__(256) k_proxy_list(const ProxyItem *__restrict_
     const int64_t o0 = far_off[item];
     const int total = (int)(far_off[item + 1] - o0);
     const int32_t *list = far_src + o0;
+    // software pipeline: the records of chunk c + 1 and the list entries of
+    // chunk c + 2 are in flight while chunk c is evaluated
+    constexpr int S = NCH_PROXY / 64;
+    int32_t jn[S], jnn[S], nn[S];
+    T4 rn[S];
+    auto load_list = [&](int b, int32_t *jj) {
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int k = b + lane + 64 * s;
+            jj[s] = k < total ? list[k] : -1;
+        }
+    };
+    auto load_recs = [&]() {
+#pragma unroll
+        for (int s = 0; s < S; s++)
+            if (jn[s] >= 0) {
+                rn[s] = src[jn[s]];
+                nn[s] = src_node(jn[s], p0, PP, recnode);
+            }
+    };
+    load_list(0, jn);
+    load_recs();
+    load_list(NCH_PROXY, jnn);
     for (int base = 0; base < total; base += NCH_PROXY) {
         const int m = min(NCH_PROXY, total - base);
 #pragma unroll
-        for (int s = 0; s < NCH_PROXY / 64; s++) {
-            const int k = lane + 64 * s;
-            if (k < m) {
-                const int32_t j = list[base + k];
-                lds[k] = to_frame(src[j], src_node(j, p0, PP, recnode), centres, ct);
-            }
-        }
+        for (int s = 0; s < S; s++)
+            if (lane + 64 * s < m) lds[lane + 64 * s] = to_frame(rn[s], nn[s], centres, ct);
         wave_sync();
+#pragma unroll
+        for (int s = 0; s < S; s++) jn[s] = jnn[s];
+        load_recs();
+        load_list(base + 2 * NCH_PROXY, jnn);
         run_chunk(m, ph, P, act, eval);
         A.flush();
         wave_sync();
