@@ -886,31 +886,45 @@ __global__ void k_combine(const int32_t *__restrict__ recid, const int32_t *__re
     if (e > 0) {
         const int ld = e - 1;
         const int64_t locd = recnode[i] - I.nbase - fssg::level_offset(ld);
-        for (int l = 0; l < e; l++) {
-            const int32_t g = (int32_t)(I.nbase + fssg::level_offset(l) + (locd >> (2 * (ld - l))));
-            // a node shared by the whole wave (the upper levels): its data is
-            // read once through scalar loads instead of 64 vector lanes
+        auto node_at = [&](int l) { return (int32_t)(I.nbase + fssg::level_offset(l) + (locd >> (2 * (ld - l)))); };
+        // node data of level l (near-field slot, first main record, interpolant
+        // slot), fetched one level ahead of its use; a node shared by the whole
+        // wave (the upper levels) is read through scalar loads
+        int32_t os, rx, sl;
+        bool uni;
+        auto fetch = [&](int32_t g) {
             const int32_t gu = __builtin_amdgcn_readfirstlane(g);
-            if (__all(g == gu)) {
-                const int32_t os = oslot[gu];
-                if (os >= 0) {
-                    const int64_t pos = os + (i - nrange[gu].x);
-                    a0 += out[pos];
-                    a1 += out[ostride + pos];
-                    a2 += out[2 * ostride + pos];
-                }
-                const int32_t sl = node_slot[gu];
-                if (sl >= 0) add_interp<D>(tinv, alpha, gu, sl, pt, normalise, a0, a1, a2);
+            uni = __all(g == gu);
+            if (uni) {
+                os = oslot[gu];
+                rx = nrange[gu].x;
+                sl = node_slot[gu];
             } else {
-                const int32_t os = oslot[g];
-                if (os >= 0) {
-                    const int64_t pos = os + (i - nrange[g].x);
-                    a0 += out[pos];
-                    a1 += out[ostride + pos];
-                    a2 += out[2 * ostride + pos];
-                }
-                const int32_t sl = node_slot[g];
-                if (sl >= 0) add_interp<D>(tinv, alpha, g, sl, pt, normalise, a0, a1, a2);
+                os = oslot[g];
+                rx = nrange[g].x;
+                sl = node_slot[g];
+            }
+        };
+        int32_t g = node_at(0);
+        fetch(g);
+        for (int l = 0; l < e; l++) {
+            const int32_t cg = g, cos = os, crx = rx, csl = sl;
+            const bool cu = uni;
+            if (l + 1 < e) {
+                g = node_at(l + 1);
+                fetch(g);
+            }
+            if (cos >= 0) {
+                const int64_t pos = cos + (i - crx);
+                a0 += out[pos];
+                a1 += out[ostride + pos];
+                a2 += out[2 * ostride + pos];
+            }
+            if (cu) {
+                const int32_t gu = __builtin_amdgcn_readfirstlane(cg), su = __builtin_amdgcn_readfirstlane(csl);
+                if (su >= 0) add_interp<D>(tinv, alpha, gu, su, pt, normalise, a0, a1, a2);
+            } else if (csl >= 0) {
+                add_interp<D>(tinv, alpha, cg, csl, pt, normalise, a0, a1, a2);
             }
         }
     }
