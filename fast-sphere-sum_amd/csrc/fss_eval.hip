@@ -366,18 +366,21 @@ __device__ inline void run_chunk_from(int lo, int fill, int ph, int P, bool act,
     for (int k = ph < lo ? ph + (lo - ph + P - 1) / P * P : ph; k < fill; k += P) eval(k);
 }
 
-// target position as a double (absolute for FP64 records, from the input
-// arrays for FP32 offset records)
-__device__ inline void abs_pos(const double4 &r, int32_t, const double *, const double *,
-                               const double *, double *o)
+// absolute position of the target record at position j: FP64 records hold
+// it, FP32 offset records come from the caller's arrays
+__device__ inline void target_pos(const double4 *__restrict__ src, const int32_t *, int64_t j,
+                                  const double *, const double *, const double *, double *o)
 {
+    const double4 r = src[j];
     o[0] = r.x;
     o[1] = r.y;
     o[2] = r.z;
 }
-__device__ inline void abs_pos(const float4 &, int32_t id, const double *__restrict__ x,
-                               const double *__restrict__ y, const double *__restrict__ z, double *o)
+__device__ inline void target_pos(const float4 *, const int32_t *__restrict__ recid, int64_t j,
+                                  const double *__restrict__ x, const double *__restrict__ y,
+                                  const double *__restrict__ z, double *o)
 {
+    const int32_t id = recid[j];
     o[0] = x[id];
     o[1] = y[id];
     o[2] = z[id];
@@ -435,8 +438,8 @@ template <int R> struct Acc<float, R> {
 // backward c_q = P_{q-1} r, r <- den_q r -- which saves R - 1 of the
 // quarter-rate v_rcp_f64 and their Newton steps (relative error of each c
 // within a few ulp more than one reciprocal per pair); a zero denominator
-// (coincident points) falls back to one reciprocal per pair.  FP32: one pair
-// at a time.
+// (coincident points) falls back to one reciprocal per pair.  FP32: the same
+// with v_rcp_f32 (no Newton step) on the offset records.
 template <int R, bool SPH, typename AccT>
 __device__ inline void near_tile(const double4 *xt, const double4 &s, AccT &A)
 {
@@ -472,10 +475,33 @@ __device__ inline void near_tile(const double4 *xt, const double4 &s, AccT &A)
 template <int R, bool SPH, typename AccT>
 __device__ inline void near_tile(const float4 *xt, const float4 &s, AccT &A)
 {
+    float d[R][3], den[R], pre[R], c[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        d[q][0] = s.x - xt[q].x;
+        d[q][1] = s.y - xt[q].y;
+        d[q][2] = s.z - xt[q].z;
+        den[q] = fmaf(d[q][0], d[q][0], fmaf(d[q][1], d[q][1], d[q][2] * d[q][2]));
+        pre[q] = q == 0 ? den[0] : pre[q - 1] * den[q];
+    }
+    if (pre[R - 1] != 0.f) {
+        float rw = s.w * rcp32(pre[R - 1]);
+#pragma unroll
+        for (int q = R - 1; q > 0; q--) {
+            c[q] = pre[q - 1] * rw;
+            rw = den[q] * rw;
+        }
+        c[0] = rw;
+    } else {
+#pragma unroll
+        for (int q = 0; q < R; q++) c[q] = s.w * rcp32(den[q]);
+    }
 #pragma unroll
     for (int q = 0; q < R; q++) {
         float *pq = A.part(q);
-        pair_eval(xt[q], s, 0.f, pq[0], pq[1], pq[2]);
+        pq[0] = fmaf(c[q], d[q][0], pq[0]);
+        pq[1] = fmaf(c[q], d[q][1], pq[1]);
+        pq[2] = fmaf(c[q], d[q][2], pq[2]);
     }
 }
 
@@ -500,8 +526,10 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
                                              const double *__restrict__ z, int64_t ostride, double *out)
 {
     using T4 = typename Rec<Real>::T4;
-    constexpr int LDS_BYTES = (sizeof(T4) * NCH > sizeof(double) * 64 * 3 * R) ? sizeof(T4) * NCH
-                                                                                 : sizeof(double) * 64 * 3 * R;
+    // staged records, or after the last chunk the phase partials [64][3R] and
+    // sums [T][3R] (T <= 64 / R + 1)
+    constexpr int RED_BYTES = sizeof(double) * (64 * 3 * R + (64 / R + 1) * 3 * R);
+    constexpr int LDS_BYTES = sizeof(T4) * NCH > RED_BYTES ? sizeof(T4) * NCH : RED_BYTES;
     __shared__ __align__(32) unsigned char smem[LDS_BYTES];
     __shared__ int32_t lid[NCH];
     T4 *lds = reinterpret_cast<T4 *>(smem);
@@ -559,37 +587,39 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
         A.flush();
         __syncthreads();
     }
+    // sum over the phases, every (group lane, target, component) by its own
+    // lane (phases in ascending order), then one lane per target for the cross
+    // product with the absolute target position
     double *red = reinterpret_cast<double *>(smem); // [64][3R], after the last chunk
+    double *sum = red + 64 * 3 * R;                 // [T][3R]
 #pragma unroll
     for (int q = 0; q < R; q++)
 #pragma unroll
         for (int c = 0; c < 3; c++) red[lane * 3 * R + 3 * q + c] = A.total(q, c);
     __syncthreads();
-    if (ph == 0) {
-#pragma unroll
-        for (int q = 0; q < R; q++) {
-            int t = tl + q * T;
-            if (t >= cnt) continue;
-            double a0 = 0, a1 = 0, a2 = 0;
-            for (int h = 0; h < P; h++) {
-                const double *rr = red + (h * T + tl) * 3 * R + 3 * q;
-                a0 += rr[0];
-                a1 += rr[1];
-                a2 += rr[2];
-            }
-            double xa[3];
-            abs_pos(xt[q], tid[q], x, y, z, xa);
-            double x0 = xa[0], x1 = xa[1], x2 = xa[2];
-            if (SPH) { // S holds half the sum (denominator |d|^2 = 2 (1 - x.y))
-                a0 *= 2.0;
-                a1 *= 2.0;
-                a2 *= 2.0;
-            }
-            int64_t pos = it.out_base + t;
-            out[pos] = x1 * a2 - x2 * a1;
-            out[ostride + pos] = x2 * a0 - x0 * a2;
-            out[2 * ostride + pos] = x0 * a1 - x1 * a0;
+    for (int o = lane; o < T * 3 * R; o += 64) {
+        const int g = o / (3 * R), r = o - g * (3 * R);
+        double a = 0.0;
+        for (int h = 0; h < P; h++) a += red[(h * T + g) * 3 * R + r];
+        sum[o] = a;
+    }
+    __syncthreads();
+    for (int o = lane; o < T * R; o += 64) {
+        const int g = o / R, q = o - g * R, t = g + q * T;
+        if (t >= cnt) continue;
+        const double *sq = sum + g * 3 * R + 3 * q;
+        double a0 = sq[0], a1 = sq[1], a2 = sq[2];
+        double xa[3];
+        target_pos(src, recid, (int64_t)it.tgt_start + t, x, y, z, xa);
+        if (SPH) { // S holds half the sum (denominator |d|^2 = 2 (1 - x.y))
+            a0 *= 2.0;
+            a1 *= 2.0;
+            a2 *= 2.0;
         }
+        const int64_t pos = it.out_base + t;
+        out[pos] = xa[1] * a2 - xa[2] * a1;
+        out[ostride + pos] = xa[2] * a0 - xa[0] * a2;
+        out[2 * ostride + pos] = xa[0] * a1 - xa[1] * a0;
     }
 }
 
