@@ -953,8 +953,20 @@ __global__ void k_combine(const int32_t *__restrict__ recid, const int32_t *__re
             if (cu) {
                 const int32_t gu = __builtin_amdgcn_readfirstlane(cg), su = __builtin_amdgcn_readfirstlane(csl);
                 if (su >= 0) add_interp<D>(tinv, alpha, gu, su, pt, normalise, a0, a1, a2);
-            } else if (csl >= 0) {
-                add_interp<D>(tinv, alpha, cg, csl, pt, normalise, a0, a1, a2);
+            } else {
+                // a few nodes per wave (the deep levels): one pass per distinct
+                // node, its data read through scalar loads (per-lane vector
+                // loads of the same 27+ doubles would load the address units)
+                bool todo = csl >= 0;
+                uint64_t m;
+                while ((m = __ballot(todo)) != 0) {
+                    const int fl = __builtin_ctzll(m);
+                    const int32_t gu = __builtin_amdgcn_readlane(cg, fl), su = __builtin_amdgcn_readlane(csl, fl);
+                    if (todo && cg == gu) {
+                        add_interp<D>(tinv, alpha, gu, su, pt, normalise, a0, a1, a2);
+                        todo = false;
+                    }
+                }
             }
         }
     }
