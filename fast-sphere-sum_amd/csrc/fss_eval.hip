@@ -282,7 +282,9 @@ __global__ void __launch_bounds__(256) k_upward(const int4 *__restrict__ items, 
     constexpr int P = npts<D>();
     // four independent waves per block, one item each (more resident waves
     // than one-wave blocks allow)
-    const int64_t wi = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // wave index within the block, made wave-uniform for the compiler (item
+    // and node data then come through scalar loads)
+    const int64_t wi = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wi >= nitems) return;
     const int4 it = items[wi];
     const int lane = threadIdx.x & 63;
@@ -293,6 +295,7 @@ __global__ void __launch_bounds__(256) k_upward(const int4 *__restrict__ items, 
     double M[P];
 #pragma unroll
     for (int k = 0; k < P; k++) M[k] = 0.0;
+#pragma unroll 4
     for (int32_t j = lane; j < it.z; j += 64) {
         int32_t pos = it.y + j;
         double pt[3], c[3], m[P], w;
@@ -327,7 +330,9 @@ __global__ void __launch_bounds__(256) k_upward_finish(const int4 *__restrict__ 
                                                        typename Rec<Real>::T4 *src)
 {
     constexpr int P = npts<D>();
-    const int64_t wi = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // wave index within the block, made wave-uniform for the compiler (item
+    // and node data then come through scalar loads)
+    const int64_t wi = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wi >= nmulti) return;
     const int4 mn = multi[wi];
     const int lane = threadIdx.x & 63;
@@ -698,7 +703,7 @@ __global__ void __launch_bounds__(256) k_proxy_list(const ProxyItem *__restrict_
     constexpr int RED_BYTES = sizeof(double) * (64 * 3 * R + NOUT);
     constexpr int LDS_BYTES = sizeof(T4) * NCH_PROXY > RED_BYTES ? sizeof(T4) * NCH_PROXY : RED_BYTES;
     __shared__ __align__(32) unsigned char smem_all[4][LDS_BYTES];
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
     unsigned char *smem = smem_all[wave];
     T4 *lds = reinterpret_cast<T4 *>(smem);
     // contiguous runs of blocks per XCD (blocks go round-robin to the 8 XCDs)
