@@ -170,15 +170,20 @@ def run_batch(args, W, fss, torch):
                            many_count=W["many_count"], precision=precision, eval_mode=fss.EVAL_BLTC,
                            assign_mode=fss.ASSIGN_ROBUST, node_rule=fss.NODES_SIN2)
 
-    # autotune the leaf size on the median instance, transfer it by occupancy
+    # autotune the leaf size on the instance that holds the median particle
+    # (the batch's work sits in its large instances; a small one is bound by
+    # launch latency and favours small leaves), transfer it by occupancy
     km = int(np.argsort(sizes)[K // 2])
     nk = int(sizes[km])
-    sl = slice(int(base[km]), int(base[km + 1]))
+    order = np.argsort(sizes)
+    kt = int(order[np.searchsorted(np.cumsum(sizes[order]), sizes.sum() / 2)])
+    nt = int(sizes[kt])
+    sl = slice(int(base[kt]), int(base[kt + 1]))
     tctx = make_ctx(fss.MIXED)
-    best, tms = tctx.autotune_levels(*[d[sl] for d in dev], 3, 7, reps=3)
-    occ = nk / (20 * 4 ** (best - 1))
+    best, tms = tctx.autotune_levels(*[d[sl] for d in dev], 3, 8, reps=3)
+    occ = nt / (20 * 4 ** (best - 1))
     levels = [fss.levels_for(int(n), occ) for n in sizes]
-    log(f"autotune on N={nk}: ms per depth {tms} -> {best} levels, leaf occupancy {occ:.1f}")
+    log(f"autotune on N={nt}: ms per depth {tms} -> {best} levels, leaf occupancy {occ:.1f}")
     del tctx
 
     ctxs = {}
