@@ -55,11 +55,12 @@ __device__ inline double half_dist(const double4 &a, const double4 &b)
     return 0.5 * sqrt(dx * dx + dy * dy + dz * dz);
 }
 
-// near field: one wave per near item, one lane per (target, phase); every
-// source range of the item (the self ranges too: j = i is part of the sum)
-// staged in LDS chunks
+// near field: one wave per near item, one lane per (target, phase); the
+// item's source list (the self ranges too: j = i is part of the sum) staged in
+// LDS chunks, every chunk's loads independent
 template <int KIND>
-__global__ void __launch_bounds__(64) k_sal_near(const NearItem *__restrict__ items, const Range *__restrict__ ranges,
+__global__ void __launch_bounds__(64) k_sal_near(const NearItem *__restrict__ items,
+                                                 const int32_t *__restrict__ near_src,
                                                  const double4 *__restrict__ src, KernelArgs ka, double *out)
 {
     __shared__ double4 lds[NCH_S];
@@ -74,31 +75,21 @@ __global__ void __launch_bounds__(64) k_sal_near(const NearItem *__restrict__ it
     const bool act = ph < P;
     const double4 xt = src[it.tgt_start + (act ? tl : 0)];
     double acc = 0.0;
-    int fill = 0;
-    auto run = [&]() {
+    const int32_t *list = near_src + it.src_off;
+    const int total = it.n_other + it.n_self;
+    for (int base = 0; base < total; base += NCH_S) {
+        const int m = min(NCH_S, total - base);
+#pragma unroll
+        for (int s = 0; s < NCH_S / 64; s++)
+            if (lane + 64 * s < m) lds[lane + 64 * s] = src[list[base + lane + 64 * s]];
+        __syncthreads();
         if (act)
-            for (int k = ph; k < fill; k += P) {
+            for (int k = ph; k < m; k += P) {
                 const double4 s = lds[k];
                 acc = fma(gfun<KIND>(ka, Ps, half_dist(xt, s)), s.w, acc);
             }
-    };
-    for (int r = it.a_beg; r < it.a_end; r++) {
-        const Range rg = ranges[r];
-        const int c = rg.count & ~FSS_SELF_BIT;
-        for (int base = 0; base < c; base += 64) {
-            const int m = min(64, c - base);
-            if (fill + m > NCH_S) {
-                __syncthreads();
-                run();
-                __syncthreads();
-                fill = 0;
-            }
-            if (lane < m) lds[fill + lane] = src[rg.start + base + lane];
-            fill += m;
-        }
+        __syncthreads();
     }
-    __syncthreads();
-    run();
     red[lane] = acc;
     __syncthreads();
     if (ph == 0 && act) {
@@ -108,32 +99,51 @@ __global__ void __launch_bounds__(64) k_sal_near(const NearItem *__restrict__ it
     }
 }
 
-// proxy values: one lane per (item, proxy), the lane walks the item's ranges
+// proxy values: one wave per proxy item, one lane per (proxy, phase); the
+// item's source list staged in LDS chunks as in k_sal_near
 template <int KIND, int D>
-__global__ void __launch_bounds__(64) k_sal_proxy(const ProxyItem *__restrict__ items, int64_t nitems,
-                                                  const Range *__restrict__ ranges, const double4 *__restrict__ src,
-                                                  int64_t p0, KernelArgs ka, double *phi)
+__global__ void __launch_bounds__(64) k_sal_proxy(const ProxyItem *__restrict__ items,
+                                                  const int64_t *__restrict__ far_off,
+                                                  const int32_t *__restrict__ far_src,
+                                                  const double4 *__restrict__ src, int64_t p0, KernelArgs ka,
+                                                  double *phi)
 {
     constexpr int PP = npts_s<D>();
-    constexpr int IPW = 64 / PP;
+    constexpr int P = 64 / PP;
+    __shared__ double4 lds[NCH_S];
+    __shared__ double red[64];
     __shared__ float Ps[KIND == 1 ? NPARAM : 1];
     stage_params<KIND>(ka, Ps);
-    const int lane = threadIdx.x;
-    const int il = lane / PP, k = lane - il * PP;
-    const int64_t item = (int64_t)blockIdx.x * IPW + il;
-    if (il >= IPW || item >= nitems) return;
+    const int64_t item = blockIdx.x;
     const ProxyItem it = items[item];
-    const double4 yk = src[p0 + (int64_t)it.node * PP + k];
+    const int lane = threadIdx.x;
+    const int ph = lane / PP, k = lane - ph * PP;
+    const bool act = ph < P;
+    const double4 yk = src[p0 + (int64_t)it.node * PP + (act ? k : 0)];
     double acc = 0.0;
-    for (int r = it.b_beg; r < it.b_end; r++) {
-        const Range rg = ranges[r];
-        const int c = rg.count & ~FSS_SELF_BIT;
-        for (int j = 0; j < c; j++) {
-            const double4 s = src[rg.start + j];
-            acc = fma(gfun<KIND>(ka, Ps, half_dist(yk, s)), s.w, acc);
-        }
+    const int64_t o0 = far_off[item];
+    const int total = (int)(far_off[item + 1] - o0);
+    const int32_t *list = far_src + o0;
+    for (int base = 0; base < total; base += NCH_S) {
+        const int m = min(NCH_S, total - base);
+#pragma unroll
+        for (int s = 0; s < NCH_S / 64; s++)
+            if (lane + 64 * s < m) lds[lane + 64 * s] = src[list[base + lane + 64 * s]];
+        __syncthreads();
+        if (act)
+            for (int kk = ph; kk < m; kk += P) {
+                const double4 s = lds[kk];
+                acc = fma(gfun<KIND>(ka, Ps, half_dist(yk, s)), s.w, acc);
+            }
+        __syncthreads();
     }
-    phi[item * PP + k] = acc;
+    red[lane] = acc;
+    __syncthreads();
+    if (ph == 0 && act) {
+        double a = 0.0;
+        for (int h = 0; h < P; h++) a += red[h * PP + k];
+        phi[item * PP + k] = a;
+    }
 }
 
 // sum the chunks of every interpolant slot in order and solve V alpha = phi
@@ -242,8 +252,19 @@ __global__ void k_sal_combine(const int32_t *__restrict__ recid, const int32_t *
             } else {
                 const int32_t os = oslot[g];
                 if (os >= 0) a += out[os + (i - nrange[g].x)];
+                // a few nodes per wave: one pass per distinct node through
+                // scalar loads of its interpolant
                 const int32_t sl = node_slot[g];
-                if (sl >= 0) a += sal_interp<D>(tinv, alpha, g, sl, pt);
+                bool todo = sl >= 0;
+                uint64_t msk;
+                while ((msk = __ballot(todo)) != 0) {
+                    const int fl = __builtin_ctzll(msk);
+                    const int32_t gs = __builtin_amdgcn_readlane(g, fl), ss = __builtin_amdgcn_readlane(sl, fl);
+                    if (todo && g == gs) {
+                        a += sal_interp<D>(tinv, alpha, gs, ss, pt);
+                        todo = false;
+                    }
+                }
             }
         }
     }
@@ -282,11 +303,11 @@ template <int KIND, int D> fss_status run(fss_ctx *ctx, const KernelArgs &ka, do
     FSS_TRY(ctx->acc3.ensure(sizeof(double) * std::max<int64_t>(1, p0)));
     if (T.n_near > 0)
         hipLaunchKernelGGL((k_sal_near<KIND>), dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
-                           T.ranges_a.as<Range>(), src, ka, ctx->out3.as<double>());
+                           T.near_src.as<int32_t>(), src, ka, ctx->out3.as<double>());
     if (T.n_proxy > 0)
-        hipLaunchKernelGGL((k_sal_proxy<KIND, D>), dim3((unsigned)div_up(T.n_proxy, 64 / p)), dim3(64), 0, s,
-                           T.proxy_items.as<ProxyItem>(), T.n_proxy, T.ranges_b.as<Range>(), src, p0, ka,
-                           ctx->phi.as<double>());
+        hipLaunchKernelGGL((k_sal_proxy<KIND, D>), dim3((unsigned)T.n_proxy), dim3(64), 0, s,
+                           T.proxy_items.as<ProxyItem>(), T.far_off.as<int64_t>(), T.far_src.as<int32_t>(), src, p0,
+                           ka, ctx->phi.as<double>());
     if (T.n_slot > 0)
         hipLaunchKernelGGL((k_sal_solve<D>), dim3((unsigned)div_up(T.n_slot, 128)), dim3(128), 0, s,
                            T.proxy_slots.as<ProxySlot>(), T.n_slot, ctx->phi.as<double>(), ctx->d_lu.as<double>(),
