@@ -1193,6 +1193,27 @@ fss_status plan_finish(fss_ctx *ctx)
         hipLaunchKernelGGL(k_near_src_fill, dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
                            T.ranges_a.as<Range>(), off.as<int64_t>(), T.near_src.as<int32_t>());
         FSS_HIP(hipGetLastError());
+        // The near kernel hands each XCD a contiguous eighth of the items;
+        // inside every eighth, windows of 512 items (about one round of the
+        // XCD's wave slots, whose sources fit its L2) go longest first, so the
+        // last wave slots to free hold short items.  Cost of an item: its
+        // sources over the phases of a 4-target tile.
+        std::vector<NearItem> it(T.n_near);
+        FSS_TRY(d2h(it.data(), T.near_items.p, sizeof(NearItem) * T.n_near, s));
+        auto cost = [](const NearItem &a) {
+            const int tt = (a.tgt_cnt + 3) / 4, ph = 64 / tt;
+            return ((int64_t)a.n_other + a.n_self + ph - 1) / ph;
+        };
+        const int64_t q8 = T.n_near >> 3, r8 = T.n_near & 7;
+        int64_t b = 0;
+        for (int x = 0; x < 8; x++) {
+            const int64_t e = b + q8 + (x < r8 ? 1 : 0);
+            for (int64_t w = b; w < e; w += 512)
+                std::stable_sort(it.begin() + w, it.begin() + std::min<int64_t>(e, w + 512),
+                                 [&](const NearItem &u, const NearItem &v) { return cost(u) > cost(v); });
+            b = e;
+        }
+        FSS_HIP(hipMemcpyAsync(T.near_items.p, it.data(), sizeof(NearItem) * T.n_near, hipMemcpyHostToDevice, s));
         FSS_HIP(hipStreamSynchronize(s));
     }
     // proxy-item source lists; far_off[n_proxy] = total
