@@ -321,6 +321,58 @@ __global__ void __launch_bounds__(256) k_upward(const int4 *__restrict__ items, 
     }
 }
 
+// Upward items of at most 32 records, two per wave (lanes 0-31 and 32-63):
+// the same moments as k_upward with half the waves; both items' node data
+// come through scalar loads.  items[0, npair) holds such items back to back.
+template <typename Real, int D>
+__global__ void __launch_bounds__(256) k_upward_pair(const int4 *__restrict__ items, int64_t nsmall,
+                                                    const int32_t *__restrict__ recnode,
+                                                    const double *__restrict__ centres,
+                                                    const double *__restrict__ tinv,
+                                                    const double *__restrict__ vinv, int64_t p0,
+                                                    typename Rec<Real>::T4 *src, double *partial)
+{
+    constexpr int P = npts<D>();
+    const int64_t wi = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t ia = 2 * wi;
+    if (ia >= nsmall) return;
+    const bool hasb = ia + 1 < nsmall;
+    const int4 A = items[ia];
+    const int4 B = hasb ? items[ia + 1] : A;
+    const int lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
+    const double *RA = tinv + 9 * (int64_t)A.x, *RB = tinv + 9 * (int64_t)B.x;
+    double Rl[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) Rl[k] = half ? RB[k] : RA[k];
+    const int4 it = half ? B : A;
+    const bool mine = half == 0 || hasb;
+    double M[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) M[k] = 0.0;
+    if (mine && hl < it.z) {
+        const int32_t pos = it.y + hl;
+        double pt[3], c[3], m[P], w;
+        rec_point(src[pos], recnode[pos], centres, pt, w);
+        bary_inv(Rl, pt, c);
+        const double is = rcp64(c[0] + c[1] + c[2]);
+        monomials<D>(c[0] * is, c[1] * is, m);
+#pragma unroll
+        for (int k = 0; k < P; k++) M[k] = m[k] * w;
+    }
+#pragma unroll
+    for (int k = 0; k < P; k++)
+        for (int off = 16; off > 0; off >>= 1) M[k] += __shfl_xor(M[k], off, 64);
+    if (!mine || hl >= P) return;
+    if (it.w < 0) {
+        write_charge<Real, D>(M, vinv, p0, it.x, hl, src);
+    } else {
+        double mk = M[0];
+#pragma unroll
+        for (int k = 1; k < P; k++) mk = hl == k ? M[k] : mk;
+        partial[(int64_t)it.w * P + hl] = mk;
+    }
+}
+
 // nodes with several items: one wave per node sums the partial moments (in
 // a fixed order) and lanes k < P write q_k
 template <typename Real, int D>
@@ -1024,9 +1076,16 @@ static fss_status upward(fss_ctx *ctx, typename Rec<Real>::T4 *src)
     FSS_TRY(ctx->upart.ensure(sizeof(double) * p * std::max<int64_t>(1, T.n_up_partial)));
     double *part = ctx->upart.as<double>();
     if (T.n_up == 0) return FSS_OK;
-    hipLaunchKernelGGL((k_upward<Real, D>), dim3((unsigned)div_up(T.n_up, 4)), dim3(256), 0, s,
-                       T.up_items.as<int4>(), T.n_up, T.recnode.as<int32_t>(), T.tri_info.as<double>(),
-                       T.tinv.as<double>(), ctx->d_vinv.as<double>(), p0, src, part);
+    // items of <= 32 records first (two per wave), then the others
+    const int64_t ns = T.n_up_small, nl = T.n_up - ns;
+    if (ns > 0)
+        hipLaunchKernelGGL((k_upward_pair<Real, D>), dim3((unsigned)div_up(div_up(ns, 2), 4)), dim3(256), 0, s,
+                           T.up_items.as<int4>(), ns, T.recnode.as<int32_t>(), T.tri_info.as<double>(),
+                           T.tinv.as<double>(), ctx->d_vinv.as<double>(), p0, src, part);
+    if (nl > 0)
+        hipLaunchKernelGGL((k_upward<Real, D>), dim3((unsigned)div_up(nl, 4)), dim3(256), 0, s,
+                           T.up_items.as<int4>() + ns, nl, T.recnode.as<int32_t>(), T.tri_info.as<double>(),
+                           T.tinv.as<double>(), ctx->d_vinv.as<double>(), p0, src, part);
     if (T.n_up_multi > 0)
         hipLaunchKernelGGL((k_upward_finish<Real, D>), dim3((unsigned)div_up(T.n_up_multi, 4)), dim3(256), 0, s,
                            T.up_multi.as<int4>(), T.n_up_multi, part, ctx->d_vinv.as<double>(), p0, src);

@@ -197,7 +197,7 @@ struct Tree {
     DevBuf charge_nodes; // int32 [n_charge] nodes whose proxies carry charges (BLTC)
     // upward pass (BLTC): items of <= FSS_UP_CHUNK records of one charge node;
     // nodes with several items sum their partial moments in a second pass
-    int64_t n_up = 0, n_up_multi = 0, n_up_partial = 0;
+    int64_t n_up = 0, n_up_multi = 0, n_up_partial = 0, n_up_small = 0; // items <= 32 records first
     DevBuf up_items;    // int4 [n_up] {node, first record, records, partial index or -1}
     DevBuf up_multi;    // int4 [n_up_multi] {node, first partial, end partial, 0}
     DevBuf tinv;        // double [9 nn]: rows r_m with barycentric c_m = r_m . p

@@ -1261,6 +1261,9 @@ fss_status plan_finish(fss_ctx *ctx)
         if (nchunk == 0) items.push_back(make_int4(g, 0, 0, -1)); // empty node: zero charges
         if (several) multi.push_back(make_int4(g, first, npart, 0));
     }
+    // items of at most 32 records first (the upward pass runs them two per wave)
+    std::stable_partition(items.begin(), items.end(), [](const int4 &q) { return q.z <= 32; });
+    T.n_up_small = std::count_if(items.begin(), items.end(), [](const int4 &q) { return q.z <= 32; });
     T.n_up = (int64_t)items.size();
     T.n_up_multi = (int64_t)multi.size();
     T.n_up_partial = npart;
