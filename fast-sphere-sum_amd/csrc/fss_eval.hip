@@ -3,28 +3,30 @@
 // Reference map (Old-BVE-code/):
 //   rhs_fast_sum  rhs_eval.h:34-45   loop over the interaction list
 //   pp            fast_rhs.h:93-118  -> k_near (class A, particle sources)
-//   cp            fast_rhs.h:173-236 -> k_proxy + k_solve (class B) and the
-//                                       interpolation in k_combine
+//   cp            fast_rhs.h:173-236 -> k_proxy_list + k_solve (class B) and
+//                                       the interpolation in k_combine
 //   rhs_func      rhs_eval.h:47-56   -> k_combine (sum over levels, -1/(4 pi), dw/dt)
 //
 // Layout in HBM (built by fss_tree.hip):
-//   src[0, p0)            particles in level order: the list of every node
-//                         (tree_tri_points) is a contiguous range
+//   src[0, p0)            records in tree order: one main record per particle,
+//                         sorted by its path, so the main members of any node
+//                         are a contiguous range; exception records after them
 //   src[p0, p0 + nn*p)    proxies of every node (static positions; weights are
 //                         the modified charges of the upward pass in BLTC)
-//   ranges                (start, count) source ranges, grouped by target node
-//                         (class A: direct targets, class B: proxy targets)
+//   near_src, far_src     source record positions of every near / proxy item
+//                         (the records of its class-A / class-B ranges)
 // Interpolation is linear, so the proxy values of all class-B interactions of
-// a target node are summed first (k_proxy, k_solve) and interpolated once per
-// particle (k_combine); the reference interpolates per interaction.
+// a target node are summed first (k_proxy_list, k_solve) and interpolated once
+// per particle (k_combine); the reference interpolates per interaction.
 //
 // Near and proxy kernels: one wave64 per work item.  Sources are staged in
-// LDS chunks of NCH records; lanes are (target group, phase): each lane owns
-// R targets (register blocking: every LDS read feeds R pair evaluations) and
-// phase ph walks the chunk entries ph, ph+P, ...; the phase partials are
-// summed through LDS in a fixed order (deterministic, no atomics).  Ranges
-// whose source node contains the target node are evaluated in a second pass
-// that skips the self pair by particle id, as the reference does.
+// LDS chunks from the item's source list; lanes are (target group, phase):
+// each lane owns R targets (register blocking: every LDS read feeds R pair
+// evaluations, with one reciprocal for the R denominators) and phase ph walks
+// the chunk entries ph, ph+P, ...; the phase partials are summed through LDS
+// in a fixed order (deterministic, no atomics).  Sources of ranges whose node
+// contains the target node come last in a near item's list and are evaluated
+// with the self pair masked by particle id, as the reference skips it.
 #include <cmath>
 
 #include "fss_geom.h"
