@@ -70,7 +70,9 @@ fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *le
     for (int k = 0; k < ninst; k++) {
         if (ns[k] < 1) return set_error("batch: every instance needs n >= 1"), FSS_ERR_INVALID;
         ctx->cfg.tree_levels = levels ? levels[k] : L0;
+        ctx->batch_part = true; // the merged plan is finished below
         fss_status st = tree_build(ctx, x + pb, y + pb, z + pb, ns[k]);
+        ctx->batch_part = false;
         ctx->cfg.tree_levels = L0;
         FSS_TRY(st);
         Tree &T = ctx->tree;

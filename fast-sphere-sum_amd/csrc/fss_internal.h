@@ -245,6 +245,7 @@ struct fss_ctx {
     fss_config cfg{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    bool batch_part = false; // tree_build of one batch instance: the merged plan is finished once
     fss::Interp interp;
     fss::Tree tree;
     fss::Timing timing;

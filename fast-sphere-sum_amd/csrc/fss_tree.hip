@@ -1090,7 +1090,7 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
     }
     FSS_HIP(hipGetLastError());
     FSS_HIP(hipStreamSynchronize(s));
-    FSS_TRY(plan_finish(ctx));
+    if (!ctx->batch_part) FSS_TRY(plan_finish(ctx));
     T.built = true;
     return FSS_OK;
 }
