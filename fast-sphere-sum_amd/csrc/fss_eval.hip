@@ -70,20 +70,10 @@ __device__ inline double dot_ref(const double4 &a, const double4 &b)
     return s;
 }
 
-// One pair evaluation.  The three components accumulate c * (x_j - x_i): the
-// caller takes x_i x S at the end (x_i x x_j = x_i x (x_j - x_i)), which keeps
-// near pairs free of cancellation.
-// FP64 near field: reference denominator 1 - x.y.
-__device__ inline void pair_eval(const double4 &xt, const double4 &s, double, double &S0, double &S1,
-                                 double &S2)
-{
-    double den = 1.0 - dot_ref(xt, s);
-    double c = s.w * rcp64(den);
-    S0 = fma(c, s.x - xt.x, S0);
-    S1 = fma(c, s.y - xt.y, S1);
-    S2 = fma(c, s.z - xt.z, S2);
-}
-
+// Pair evaluations accumulate c * (x_j - x_i): the caller takes x_i x S at the
+// end (x_i x x_j = x_i x (x_j - x_i)), which keeps near pairs free of
+// cancellation.  FP64 pairs are evaluated in near_tile, near_pair_masked and
+// far_tile below.
 // FP32: records are offsets from a common node centre and carry the doubled
 // weight 2 w; 2 (1 - x.y) = |d|^2 + off2, where off2 = 1 - |y|^2 for a target
 // off the sphere (planar proxy), so c = 2 w / (|d|^2 + off2) = w / (1 - x.y).
@@ -126,17 +116,7 @@ __device__ inline void near_pair_masked(const float4 &xt, const float4 &s, bool 
     S2 = fmaf(c, dz, S2);
 }
 
-// FP64 far field (well separated: no cancellation): y_k x sum c_j x_j
-__device__ inline void pair_far(const double4 &xt, const double4 &s, double, double &S0, double &S1,
-                                double &S2)
-{
-    double den = 1.0 - (xt.x * s.x + xt.y * s.y + xt.z * s.z);
-    double c = s.w * rcp64(den);
-    S0 = fma(c, s.x, S0);
-    S1 = fma(c, s.y, S1);
-    S2 = fma(c, s.z, S2);
-}
-
+// FP32 far field: the same offset form
 __device__ inline void pair_far(const float4 &xt, const float4 &s, float off, float &S0, float &S1,
                                 float &S2)
 {
