@@ -2,7 +2,7 @@
 """bench.py -- particle velocities per second for one RK4-stage velocity
 evaluation on one MI355X (libfss, HIP), with the CPU oracle timed beside it.
 
-Default workload: icosahedral-tree barycentric-Lagrange
+Default workload (c2): icosahedral-tree barycentric-Lagrange
 treecode in FP64 on the N = 655,362 vertices of a level-8 icosahedral grid,
 dual-Voronoi areas, Rossby-Haurwitz vorticity (w0 = K = 0.5, Omega = 2 pi),
 5 refinements of the 20 faces (6 tree levels), degree 2 (6 proxies),
@@ -14,7 +14,11 @@ on 1e4 seeded targets.
 reference-dispatch (parity) tree, the mixed-precision tree, both direct sums,
 the roofline of the dominant kernel (k_near64) and the CPU baseline.
 
-  python bench.py [--steps K] [--warmup W] [--config c2|c3] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--config c2|c3|c4|c5] [--no-cpu]
+
+c3: the same treecode in mixed precision on the level-10 grid (N = 10,485,762);
+c4: 64 batched instances (18.0 M particles) stepped by RK4 from a hipGraph;
+c5: the SAL convolution by the treecode on the level-9 grid.
 """
 import argparse
 import json
