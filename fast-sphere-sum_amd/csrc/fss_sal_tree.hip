@@ -10,7 +10,7 @@
 // records; G is one of
 //   KIND 0  the tabulated series (cubic interpolation, the fast fallback),
 //   KIND 1  the learned kernel (analytic singular part + network),
-//   KIND 2  the series itself by Clenshaw summation (the judge).
+//   KIND 2  the series itself by Clenshaw summation (the accuracy reference).
 #include <hip/hip_runtime.h>
 
 #include "fss_geom.h"

@@ -50,7 +50,7 @@ typedef enum { FSS_ASSIGN_PARITY = 0, FSS_ASSIGN_ROBUST = 1 } fss_assign_mode;
  * raw barycentric coordinates (fast_rhs.h:173-236).  BLTC is the project's
  * barycentric-Lagrange treecode: pp, pc (source proxies with modified
  * charges), cp, cc, proxies on the sphere, normalised barycentric
- * coordinates; judged against the direct sum. */
+ * coordinates; its accuracy is measured against the direct sum. */
 typedef enum { FSS_EVAL_PARITY = 0, FSS_EVAL_BLTC = 1 } fss_eval_mode;
 
 /* Interpolation node rule (interp_utils.h:12-31).  REFERENCE runs fekete_init
