@@ -923,8 +923,8 @@ __device__ inline void add_interp(const double *__restrict__ tinv, const double 
 // (output slot of its node + its offset in the node's range) and the
 // interpolants of the nodes on that path, at the position held by the record;
 // the rare memberships outside the main path come from the per-level maps.
-// The sums go to acc (tree order, coalesced); k_scatter moves them to the
-// caller's order.
+// The sums, with the rhs_func scaling (rhs_eval.h:54-55), go straight to the
+// particle's place in the caller's arrays.
 template <typename Real, int D>
 __global__ void k_combine(const int32_t *__restrict__ recid, const int32_t *__restrict__ recnode,
                           const uint8_t *__restrict__ rinfo, const int4 *__restrict__ nrange,
@@ -933,7 +933,8 @@ __global__ void k_combine(const int32_t *__restrict__ recid, const int32_t *__re
                           const double *__restrict__ alpha, const double *__restrict__ tinv,
                           const typename Rec<Real>::T4 *__restrict__ src, const double *__restrict__ centres,
                           const Instance *__restrict__ inst, int ninst, int64_t nrec, int64_t ostride,
-                          const double *__restrict__ out, int normalise, double *acc)
+                          const double *__restrict__ out, int normalise, double scale, double omega,
+                          double *u, double *v, double *w, double *dvort)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrec) return;
@@ -1025,23 +1026,11 @@ __global__ void k_combine(const int32_t *__restrict__ recid, const int32_t *__re
             if (sl >= 0) add_interp<D>(tinv, alpha, g, sl, pt, normalise, a0, a1, a2);
         }
     }
-    acc[i] = a0;
-    acc[nrec + i] = a1;
-    acc[2 * nrec + i] = a2;
-}
-
-// caller's order and the rhs_func scaling (rhs_eval.h:54-55)
-__global__ void k_scatter(const int32_t *__restrict__ ipos, int64_t n, const double *__restrict__ acc, int64_t nrec,
-                          double scale, double omega, double *u, double *v, double *w, double *dvort)
-{
-    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const int64_t i = ipos[id];
-    const double a2 = acc[2 * nrec + i] * scale;
-    u[id] = acc[i] * scale;
-    v[id] = acc[nrec + i] * scale;
-    w[id] = a2;
-    dvort[id] = -2 * omega * a2;
+    const double s2 = a2 * scale;
+    u[id] = a0 * scale;
+    v[id] = a1 * scale;
+    w[id] = s2;
+    dvort[id] = -2 * omega * s2;
 }
 
 // ------------------------------------------------------------------
@@ -1143,23 +1132,22 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
         hipLaunchKernelGGL((k_solve<D>), dim3((unsigned)div_up(3 * T.n_slot, 128)), dim3(128), 0, s,
                            T.proxy_slots.as<ProxySlot>(), T.n_slot, ctx->phi.as<double>(),
                            ctx->d_lu.as<double>(), ctx->d_ipiv.as<int32_t>(), ctx->alpha.as<double>());
-    FSS_TRY(ctx->acc3.ensure(sizeof(double) * 3 * std::max<int64_t>(1, p0)));
     if (mixed)
         hipLaunchKernelGGL((k_combine<float, D>), dim3((unsigned)div_up(p0, 256)), dim3(256), 0, s,
                            T.recid.as<int32_t>(), T.recnode.as<int32_t>(), T.rinfo.as<uint8_t>(), T.nrange.as<int4>(),
                            T.oslot.as<int32_t>(), T.node_slot.as<int32_t>(), T.nodeof.as<int32_t>(),
                            T.inv_active.as<int32_t>(), ctx->alpha.as<double>(), T.tinv.as<double>(),
                            ctx->src32.as<float4>(), T.tri_info.as<double>(), T.inst.as<Instance>(), T.ninst, p0, ost,
-                           ctx->out3.as<double>(), bltc ? 1 : 0, ctx->acc3.as<double>());
+                           ctx->out3.as<double>(), bltc ? 1 : 0, scale, ctx->cfg.omega, out->u, out->v, out->w,
+                           out->dvort);
     else
         hipLaunchKernelGGL((k_combine<double, D>), dim3((unsigned)div_up(p0, 256)), dim3(256), 0, s,
                            T.recid.as<int32_t>(), T.recnode.as<int32_t>(), T.rinfo.as<uint8_t>(), T.nrange.as<int4>(),
                            T.oslot.as<int32_t>(), T.node_slot.as<int32_t>(), T.nodeof.as<int32_t>(),
                            T.inv_active.as<int32_t>(), ctx->alpha.as<double>(), T.tinv.as<double>(),
                            ctx->src64.as<double4>(), T.tri_info.as<double>(), T.inst.as<Instance>(), T.ninst, p0, ost,
-                           ctx->out3.as<double>(), bltc ? 1 : 0, ctx->acc3.as<double>());
-    hipLaunchKernelGGL(k_scatter, dim3((unsigned)div_up(T.n, 256)), dim3(256), 0, s, T.ipos.as<int32_t>(), T.n,
-                       ctx->acc3.as<double>(), p0, scale, ctx->cfg.omega, out->u, out->v, out->w, out->dvort);
+                           ctx->out3.as<double>(), bltc ? 1 : 0, scale, ctx->cfg.omega, out->u, out->v, out->w,
+                           out->dvort);
     if (ev) FSS_HIP(hipEventRecord(ev[3], s));
     FSS_HIP(hipGetLastError());
     return FSS_OK;
