@@ -924,8 +924,8 @@ static void k_cc_ref(double *mod, const orc_state *s, const orc_tree *t, const i
 /* ---- barycentric-Lagrange treecode (project design, no reference
  * counterpart: proxies are the gnomonic images of the interpolation nodes on
  * the sphere, particles use normalised barycentric coordinates, the Lagrange
- * basis is L_k(b) = sum_idx Vinv[idx][k] m_idx(b)).  Judged against the
- * direct sum. ---- */
+ * basis is L_k(b) = sum_idx Vinv[idx][k] m_idx(b)).  Its accuracy is
+ * measured against the direct sum. ---- */
 static void sph_proxy(const orc_tree *t, const double *v1, const double *v2, const double *v3, int k,
                       double *out)
 {
