@@ -565,13 +565,15 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
                                              const double *__restrict__ z, int64_t ostride, double *out)
 {
     using T4 = typename Rec<Real>::T4;
-    // staged records, or after the last chunk the phase partials [64][3R] and
-    // sums [T][3R] (T <= 64 / R + 1)
+    // staged records and their particle ids, or after the last chunk the
+    // phase partials [64][3R] and sums [T][3R] (T <= 64 / R + 1); one buffer
+    // for both keeps the FP32 kernel at 5 KiB, 8 blocks of one wave per SIMD
+    constexpr int STAGE_BYTES = (sizeof(T4) + sizeof(int32_t)) * NCH;
     constexpr int RED_BYTES = sizeof(double) * (64 * 3 * R + (64 / R + 1) * 3 * R);
-    constexpr int LDS_BYTES = sizeof(T4) * NCH > RED_BYTES ? sizeof(T4) * NCH : RED_BYTES;
+    constexpr int LDS_BYTES = STAGE_BYTES > RED_BYTES ? STAGE_BYTES : RED_BYTES;
     __shared__ __align__(32) unsigned char smem[LDS_BYTES];
-    __shared__ int32_t lid[NCH];
     T4 *lds = reinterpret_cast<T4 *>(smem);
+    int32_t *lid = reinterpret_cast<int32_t *>(smem + sizeof(T4) * NCH);
     // Blocks go round-robin to the 8 XCDs: give every XCD a contiguous run of
     // items (neighbouring leaves, largely the same sources) for its own L2.
     const int nb = gridDim.x, q8 = nb >> 3, r8 = nb & 7, xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
