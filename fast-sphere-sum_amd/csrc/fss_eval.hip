@@ -567,7 +567,8 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
     using T4 = typename Rec<Real>::T4;
     // staged records and their particle ids, or after the last chunk the
     // phase partials [64][3R] and sums [T][3R] (T <= 64 / R + 1); one buffer
-    // for both keeps the FP32 kernel at 5 KiB, 8 blocks of one wave per SIMD
+    // for both keeps the FP32 two-target kernel at 5 KiB, so 32 one-wave
+    // blocks (8 per SIMD) fit in the CU's 160 KiB
     constexpr int STAGE_BYTES = (sizeof(T4) + sizeof(int32_t)) * NCH;
     constexpr int RED_BYTES = sizeof(double) * (64 * 3 * R + (64 / R + 1) * 3 * R);
     constexpr int LDS_BYTES = STAGE_BYTES > RED_BYTES ? STAGE_BYTES : RED_BYTES;
