@@ -22,7 +22,7 @@
  *     reference linked macOS Accelerate, whose internal order is unpublished;
  *     netlib order is the pinned definition used here.
  *
- * Pinning (how each function is checked, see tests/test_oracle_*.py):
+ * Pinning (how each function is checked, see tests/test_oracle.py):
  *   direct sum          brute force in long double on small inputs, the
  *                       antisymmetry identity sum_i w_i u_i = 0, u.x = 0,
  *                       rotation equivariance, closed-form velocity of a
@@ -40,6 +40,12 @@
  *   interpolation       degree-2 node set = uniform lattice, exact reproduction
  *                       of polynomials of degree <= d, basis independence
  *   tree sum            converges to the direct sum as theta -> 0 / degree up
+ *   reference cp        converges with degree to the kernel at each target's
+ *                       planar image c0 v1 + c1 v2 + (1 - c0 - c1) v3 (raw
+ *                       barycentric c); equals an independent numpy
+ *                       interpolant through the planar-proxy values; mutants
+ *                       (proxies on the sphere, normalised coordinates,
+ *                       (c1, c2) as (s, t)) are rejected
  *   RK4 / projection    absolute vorticity conserved by projection, 4th-order
  *                       convergence on solid-body rotation
  */
@@ -1069,26 +1075,29 @@ static void finish(int64_t n, const double *mod, double Omega, double *u, double
     }
 }
 
-/* rhs_eval.h:34-45 + 47-56 for use_fast.  nthreads > 1 splits the list in
- * contiguous chunks with private accumulators (summation order changes,
- * rounding only). */
-ORC_API void orc_tree_rhs(void *h, const double *x, const double *y, const double *z,
-                          const double *vort, const double *area, double Omega, int nthreads,
-                          double *u, double *v, double *w, double *dvort)
+/* rhs_eval.h:34-45 + 47-56 for use_fast, restricted to the interactions i
+ * with take[i] != 0 (all when take is NULL), so a check can isolate one
+ * kernel's contribution.  nthreads > 1 splits the list in contiguous chunks
+ * with private accumulators (summation order changes, rounding only). */
+ORC_API void orc_tree_rhs_select(void *h, const double *x, const double *y, const double *z,
+                                 const double *vort, const double *area, double Omega, int nthreads,
+                                 const uint8_t *take, double *u, double *v, double *w, double *dvort)
 {
     orc_tree *t = (orc_tree *)h;
     orc_state s = {x, y, z, vort, area};
     int64_t n = t->n;
     double *mod = (double *)calloc(3 * n, sizeof(double));
+#define ORC_TAKE(i) (!take || take[i])
     if (nthreads <= 1) {
-        for (int64_t i = 0; i < t->ninter; i++) dispatch(mod, &s, t, t->inter + 7 * i);
+        for (int64_t i = 0; i < t->ninter; i++)
+            if (ORC_TAKE(i)) dispatch(mod, &s, t, t->inter + 7 * i);
     } else {
 #ifdef _OPENMP
         double **priv = (double **)malloc(sizeof(double *) * nthreads);
         for (int k = 0; k < nthreads; k++) priv[k] = (double *)calloc(3 * n, sizeof(double));
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
         for (int64_t i = 0; i < t->ninter; i++)
-            dispatch(priv[omp_get_thread_num()], &s, t, t->inter + 7 * i);
+            if (ORC_TAKE(i)) dispatch(priv[omp_get_thread_num()], &s, t, t->inter + 7 * i);
 #pragma omp parallel for num_threads(nthreads) schedule(static)
         for (int64_t i = 0; i < 3 * n; i++) {
             double a = 0;
@@ -1098,11 +1107,20 @@ ORC_API void orc_tree_rhs(void *h, const double *x, const double *y, const doubl
         for (int k = 0; k < nthreads; k++) free(priv[k]);
         free(priv);
 #else
-        for (int64_t i = 0; i < t->ninter; i++) dispatch(mod, &s, t, t->inter + 7 * i);
+        for (int64_t i = 0; i < t->ninter; i++)
+            if (ORC_TAKE(i)) dispatch(mod, &s, t, t->inter + 7 * i);
 #endif
     }
+#undef ORC_TAKE
     finish(n, mod, Omega, u, v, w, dvort);
     free(mod);
+}
+
+ORC_API void orc_tree_rhs(void *h, const double *x, const double *y, const double *z,
+                          const double *vort, const double *area, double Omega, int nthreads,
+                          double *u, double *v, double *w, double *dvort)
+{
+    orc_tree_rhs_select(h, x, y, z, vort, area, Omega, nthreads, NULL, u, v, w, dvort);
 }
 
 /* rhs_eval.h:15-32 + 47-56 (direct).  Targets idx[0..m) (all if idx == NULL). */

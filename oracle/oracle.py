@@ -36,6 +36,13 @@ def lib():
     path = os.path.join(_HERE, "liboracle.so")
     if not os.path.exists(path):
         build()
+    _LIB = load(path)
+    return _LIB
+
+
+def load(path):
+    """A CDLL of an oracle build at `path` with its signatures declared (a
+    separately built variant can be checked side by side with the default)."""
     L = C.CDLL(path)
     vp, i64, i32, f64 = C.c_void_p, C.c_int64, C.c_int, C.c_double
     L.orc_level_offset.restype = i64
@@ -79,6 +86,8 @@ def lib():
         getattr(L, nm).argtypes = [vp]
     L.orc_tree_rhs.restype = None
     L.orc_tree_rhs.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, f64, i32, _dp, _dp, _dp, _dp]
+    L.orc_tree_rhs_select.restype = None
+    L.orc_tree_rhs_select.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, f64, i32, _bp, _dp, _dp, _dp, _dp]
     L.orc_rhs_direct.restype = None
     L.orc_rhs_direct.argtypes = [i64, _dp, _dp, _dp, _dp, _dp, f64, C.c_void_p, i64, i32,
                                  _dp, _dp, _dp, _dp]
@@ -96,7 +105,6 @@ def lib():
     L.orc_rk4_step_unprojected.argtypes = [vp, i64, _dp, _dp, _dp, _dp, _dp, f64, f64, i32]
     L.orc_rk4_step.restype = None
     L.orc_rk4_step.argtypes = [vp, i64, _dp, _dp, _dp, _dp, _dp, f64, f64, i32]
-    _LIB = L
     return L
 
 
@@ -169,8 +177,8 @@ class Tree:
     (new_dynamics.cpp:76-94 call order) on particles x, y, z."""
 
     def __init__(self, x, y, z, tree_levels, theta=0.7, many_count=10, degree=2,
-                 assign=ASSIGN_PARITY, eval_mode=EVAL_PARITY, node_rule=0):
-        L = lib()
+                 assign=ASSIGN_PARITY, eval_mode=EVAL_PARITY, node_rule=0, library=None):
+        L = self._L = library if library is not None else lib()
         self.x, self.y, self.z = _c(x), _c(y), _c(z)
         self.n = self.x.shape[0]
         self.L = tree_levels
@@ -182,59 +190,59 @@ class Tree:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_tree_destroy(self.h)
+            self._L.orc_tree_destroy(self.h)
             self.h = None
 
     @property
     def ninter(self):
-        return lib().orc_tree_ninter(self.h)
+        return self._L.orc_tree_ninter(self.h)
 
     @property
     def mac_margin(self):
-        return lib().orc_tree_margin(self.h)
+        return self._L.orc_tree_margin(self.h)
 
     def interactions(self):
         """int32 [M, 7]: lev_t, lev_s, idx_t, idx_s, n_t, n_s, type (0 pp,1 pc,2 cp,3 cc)."""
         m = self.ninter
-        p = lib().orc_tree_inter(self.h)
+        p = self._L.orc_tree_inter(self.h)
         return np.ctypeslib.as_array(p, shape=(m * 7,)).reshape(m, 7).copy()
 
     def loc(self):
-        p = lib().orc_tree_loc(self.h)
+        p = self._L.orc_tree_loc(self.h)
         return np.ctypeslib.as_array(p, shape=(self.L * self.n,)).reshape(self.L, self.n).copy()
 
     def found(self):
-        p = lib().orc_tree_found(self.h)
+        p = self._L.orc_tree_found(self.h)
         return np.ctypeslib.as_array(p, shape=(self.L * self.n,)).reshape(self.L, self.n).astype(bool)
 
     def counts(self):
         T = level_offset(self.L)
-        return np.ctypeslib.as_array(lib().orc_tree_counts(self.h), shape=(T,)).copy()
+        return np.ctypeslib.as_array(self._L.orc_tree_counts(self.h), shape=(T,)).copy()
 
     def offsets(self):
         T = level_offset(self.L)
-        return np.ctypeslib.as_array(lib().orc_tree_offsets(self.h), shape=(T,)).copy()
+        return np.ctypeslib.as_array(self._L.orc_tree_offsets(self.h), shape=(T,)).copy()
 
     def level_base(self):
-        return np.ctypeslib.as_array(lib().orc_tree_level_base(self.h), shape=(self.L + 1,)).copy()
+        return np.ctypeslib.as_array(self._L.orc_tree_level_base(self.h), shape=(self.L + 1,)).copy()
 
     def ids(self):
         tot = int(self.level_base()[-1])
         if tot == 0:
             return np.zeros(0, dtype=np.int32)
-        return np.ctypeslib.as_array(lib().orc_tree_ids(self.h), shape=(tot,)).copy()
+        return np.ctypeslib.as_array(self._L.orc_tree_ids(self.h), shape=(tot,)).copy()
 
     def tri_info(self):
         T = level_offset(self.L + 1)
-        return np.ctypeslib.as_array(lib().orc_tree_tri_info(self.h), shape=(4 * T,)).reshape(T, 4).copy()
+        return np.ctypeslib.as_array(self._L.orc_tree_tri_info(self.h), shape=(4 * T,)).reshape(T, 4).copy()
 
     def tri_verts(self):
         T = level_offset(self.L + 1)
-        return np.ctypeslib.as_array(lib().orc_tree_tri_verts(self.h), shape=(3 * T,)).reshape(T, 3).copy()
+        return np.ctypeslib.as_array(self._L.orc_tree_tri_verts(self.h), shape=(3 * T,)).reshape(T, 3).copy()
 
     def verts(self):
-        nv = lib().orc_tree_nverts(self.h)
-        return np.ctypeslib.as_array(lib().orc_tree_verts(self.h), shape=(3 * nv,)).reshape(nv, 3).copy()
+        nv = self._L.orc_tree_nverts(self.h)
+        return np.ctypeslib.as_array(self._L.orc_tree_verts(self.h), shape=(3 * nv,)).reshape(nv, 3).copy()
 
     def node_list(self, lev, idx):
         g = level_offset(lev) + idx
@@ -242,11 +250,19 @@ class Tree:
         o = self.offsets()[g]
         return self.ids()[o:o + c]
 
-    def rhs(self, vort, area, omega=2 * np.pi, nthreads=1):
-        """rhs_func with use_fast (rhs_eval.h:47-56) -> (u, v, w, dvort)."""
+    def rhs(self, vort, area, omega=2 * np.pi, nthreads=1, select=None):
+        """rhs_func with use_fast (rhs_eval.h:47-56) -> (u, v, w, dvort).
+        select (optional): boolean per interaction (rows of interactions());
+        only the selected interactions contribute."""
         n = self.n
         out = [np.zeros(n) for _ in range(4)]
-        lib().orc_tree_rhs(self.h, self.x, self.y, self.z, _c(vort), _c(area), omega, nthreads, *out)
+        if select is None:
+            self._L.orc_tree_rhs(self.h, self.x, self.y, self.z, _c(vort), _c(area), omega, nthreads, *out)
+        else:
+            take = _c(np.asarray(select, dtype=bool), np.uint8)
+            assert take.shape == (self.ninter,)
+            self._L.orc_tree_rhs_select(self.h, self.x, self.y, self.z, _c(vort), _c(area), omega, nthreads,
+                                      take, *out)
         return tuple(out)
 
 

@@ -9,6 +9,7 @@ independent geometric tests and exact-coverage checks.  A plausible mistake
 at least one of them.
 """
 import itertools
+import os
 
 import numpy as np
 import pytest
@@ -378,13 +379,155 @@ def test_bltc_converges_with_degree(orc, grid5):
     assert errs[2] < 1e-5
 
 
-def test_reference_cc_and_dispatch_variants_are_finite(orc, grid5):
+def test_reference_cc_and_dispatch_variants_are_close_to_direct(orc, grid5):
+    # level-5 grid, 5 levels, theta 0.7, degree-2 lattice: measured 6.56e-2
+    # (reference dispatch pp/cp) and 1.53e-1 (the fixed pc/cc variant)
     x, y, z, vort, area, _ = grid5
     ref = np.stack(orc.rhs_direct(x, y, z, vort, area)[:3], 1)
-    for mode in (orc.EVAL_PARITY, orc.EVAL_REFCC):
+    for mode, bound in ((orc.EVAL_PARITY, 0.075), (orc.EVAL_REFCC, 0.17)):
         T = orc.Tree(x, y, z, 5, theta=0.7, assign=1, eval_mode=mode)
         e = rel_l2(np.stack(T.rhs(vort, area, nthreads=8)[:3], 1), ref)
-        assert np.isfinite(e) and e < 0.2
+        assert np.isfinite(e) and e < bound, (mode, e)
+
+
+# ------------------------------------------------- the reference cp (fast_rhs.h:173-236)
+#
+# cp interpolates f(u, v) = sum_s K(P(u, v), x_s) w_s, P(u, v) = u v1 + v v2 +
+# (1 - u - v) v3 (planar proxies, :188-200), from the p node values and
+# evaluates the polynomial at the target's raw barycentric (c0, c1)
+# (:231-234), so as the degree grows the result tends to f(c0, c1): the kernel
+# at the planar image c0 v1 + c1 v2 + (1 - c0 - c1) v3 of the target.  A point
+# on the sphere has c0 + c1 + c2 > 1, so (c0, c1) lies outside the node
+# triangle (by up to 12% on level 0): there the polynomial extrapolates and
+# the convergence stalls near 1e-3; interactions whose target node is on
+# level >= 2 extrapolate by < 4% and converge.
+
+def _planar_image_sum(orc, T, P3, w, rows, variant="ref"):
+    """-1/(4 pi) sum over interactions `rows` of K(y_t, x_s) w_s with y_t the
+    planar image of target t in the interaction's target triangle; variants
+    model plausible mistakes (normalised coordinates, (c1, c2) taken as (s, t),
+    proxies projected onto the sphere)."""
+    V, TV = T.verts(), T.tri_verts()
+    mod = np.zeros((len(P3), 3))
+    for r in rows:
+        v1, v2, v3 = V[TV[orc.level_offset(r[0]) + r[2]]]
+        tl, sl = T.node_list(r[0], r[2]), T.node_list(r[1], r[3])
+        c = np.linalg.solve(np.stack([v1, v2, v3], 1), P3[tl].T).T  # raw barycentric
+        if variant == "normalised":
+            c = c / c.sum(1)[:, None]
+        elif variant == "c12":
+            c = c[:, 1:]
+        y = c[:, :1] * v1 + c[:, 1:2] * v2 + (1 - c[:, :1] - c[:, 1:2]) * v3
+        if variant == "sphere":
+            y = y / np.linalg.norm(y, axis=1)[:, None]
+        S = P3[sl]
+        mod[tl] += np.einsum("tsk,ts->tk", np.cross(y[:, None, :], S[None]), w[sl][None] / (1 - y @ S.T))
+    return mod * (-1 / (4 * np.pi))
+
+
+def _planar_interpolant_sum(orc, T, P3, w, rows, degree, rule):
+    """The same interactions by an independent interpolant: the degree-d
+    polynomial through the kernel sums at the planar nodes, built in a
+    Chebyshev product basis by numpy, evaluated at raw (c0, c1)."""
+    V, TV = T.verts(), T.tri_verts()
+    pts = orc.interp_nodes(degree, rule)
+
+    def basis(s, t):
+        Ts = np.polynomial.chebyshev.chebvander(2 * s - 1, degree)
+        Tt = np.polynomial.chebyshev.chebvander(2 * t - 1, degree)
+        return np.stack([Ts[:, a] * Tt[:, b] for a in range(degree + 1) for b in range(degree + 1 - a)], -1)
+
+    B = basis(pts[:, 0], pts[:, 1])
+    mod = np.zeros((len(P3), 3))
+    for r in rows:
+        v1, v2, v3 = V[TV[orc.level_offset(r[0]) + r[2]]]
+        tl, sl = T.node_list(r[0], r[2]), T.node_list(r[1], r[3])
+        q = pts[:, :1] * v1 + pts[:, 1:2] * v2 + (1 - pts[:, :1] - pts[:, 1:2]) * v3
+        S = P3[sl]
+        fq = np.einsum("tsk,ts->tk", np.cross(q[:, None, :], S[None]), w[sl][None] / (1 - q @ S.T))
+        c = np.linalg.solve(np.stack([v1, v2, v3], 1), P3[tl].T).T
+        mod[tl] += basis(c[:, 0], c[:, 1]) @ np.linalg.solve(B, fq)
+    return mod * (-1 / (4 * np.pi))
+
+
+def _cp_pin_errors(orc, fsslib, library=None):
+    """(convergence errors at degrees 2, 4, 6, 8 on levels >= 2, worst
+    interpolant mismatch over all levels) for an oracle build."""
+    x, y, z, vort, area, _ = icos_case(fsslib, 4)
+    P3, w = np.stack([x, y, z], 1), vort * area
+    conv, interp = [], 0.0
+    for deg in (2, 4, 6, 8):
+        rule = 0 if deg == 2 else 1  # degree 2: the reference lattice
+        T = orc.Tree(x, y, z, 4, theta=0.7, degree=deg, assign=1, eval_mode=orc.EVAL_PARITY,
+                     node_rule=rule, library=library)
+        it = T.interactions()
+        far = it[:, 6] >= 2  # cp, and cc dispatched to cp (rhs_eval.h:41)
+        deep = far & (it[:, 0] >= 2)
+        got = np.stack(T.rhs(vort, area, nthreads=8, select=deep)[:3], 1)
+        conv.append(rel_l2(got, _planar_image_sum(orc, T, P3, w, it[deep])))
+        if deg in (2, 4):
+            got = np.stack(T.rhs(vort, area, nthreads=8, select=far)[:3], 1)
+            interp = max(interp, rel_l2(got, _planar_interpolant_sum(orc, T, P3, w, it[far], deg, rule)))
+    return conv, interp
+
+
+def _cp_pin_holds(conv, interp):
+    # measured: 2.45e-2, 1.71e-3, 2.15e-4, 3.98e-5 and interpolant 4.6e-16; the
+    # mutants of test_cp_pin_fails_on_mutated_oracle stall at 6.7e-2, 6.8e-3 and
+    # 6.3e-1 with interpolant mismatches of 9.7e-2, 6.3e-2 and 3.5e-1
+    falls = all(b < a / 4 for a, b in zip(conv, conv[1:]))
+    return falls and conv[-1] < 8e-5 and interp < 1e-11
+
+
+def test_reference_cp_converges_to_planar_image_kernel(orc, fsslib):
+    conv, interp = _cp_pin_errors(orc, fsslib)
+    assert _cp_pin_holds(conv, interp), (conv, interp)
+
+
+def test_cp_pin_separates_plausible_mistakes(orc, fsslib, grid5):
+    # the limit the pin converges to is far from the kernel at the points a
+    # normalised-coordinate, (c1, c2) or projected-proxy restatement would use
+    x, y, z, vort, area, _ = icos_case(fsslib, 4)
+    P3, w = np.stack([x, y, z], 1), vort * area
+    T = orc.Tree(x, y, z, 4, theta=0.7, degree=8, assign=1, eval_mode=orc.EVAL_PARITY, node_rule=1)
+    it = T.interactions()
+    rows = it[(it[:, 6] >= 2) & (it[:, 0] >= 2)]
+    ref = _planar_image_sum(orc, T, P3, w, rows)
+    for v in ("normalised", "c12", "sphere"):
+        assert rel_l2(_planar_image_sum(orc, T, P3, w, rows, v), ref) > 5e-3, v
+
+
+_CP_MUTATIONS = {
+    # proxies projected onto the sphere instead of left in the triangle's plane
+    "sphere_proxies": ("planar_proxy(v1, v2, v3, t->pts[3 * i], t->pts[3 * i + 1], cur[i]);",
+                       "{ planar_proxy(v1, v2, v3, t->pts[3 * i], t->pts[3 * i + 1], cur[i]); "
+                       "project_to_sphere(cur[i], 1.0); }"),
+    # normalised instead of raw barycentric coordinates
+    "normalised_coords": ("barycoords(v1, v2, v3, pt, bc);",
+                          "barycoords(v1, v2, v3, pt, bc); { double sb = bc[0] + bc[1] + bc[2]; "
+                          "bc[0] /= sb; bc[1] /= sb; bc[2] /= sb; }"),
+    # (c1, c2) taken as (s, t)
+    "c1_c2": ("interp_eval(rhs + c * p, bc[0], bc[1], t->degree)",
+              "interp_eval(rhs + c * p, bc[1], bc[2], t->degree)"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(_CP_MUTATIONS))
+def test_cp_pin_fails_on_mutated_oracle(orc, fsslib, tmp_path, name):
+    """Plants each mistake in a copy of the oracle's cp (built in tmp_path on
+    the CPU) and checks that the pin rejects it."""
+    import subprocess
+    src = open(os.path.join(os.path.dirname(orc.__file__), "fss_oracle.c")).read()
+    a, b = src.index("static void k_cp_ref("), src.index("static void k_pc_ref(")
+    old, new = _CP_MUTATIONS[name]
+    body = src[a:b]
+    assert body.count(old) == 1
+    c_file, so = tmp_path / "mutant.c", tmp_path / "libmutant.so"
+    c_file.write_text(src[:a] + body.replace(old, new) + src[b:])
+    subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=gnu11", "-shared",
+                    "-o", str(so), str(c_file), "-lm"], check=True)
+    conv, interp = _cp_pin_errors(orc, fsslib, library=orc.load(str(so)))
+    assert not _cp_pin_holds(conv, interp), (name, conv, interp)
 
 
 # ---------------------------------------------------------------- RK4 / projection
