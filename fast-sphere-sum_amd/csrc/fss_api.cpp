@@ -2,6 +2,7 @@
 // interpolation constants and argument checking.  No exception crosses the ABI.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -17,10 +18,15 @@ static thread_local std::string g_err;
 
 void set_error(const std::string &msg) { g_err = msg; }
 
+static std::atomic<uint64_t> g_alloc_epoch{0};
+
+uint64_t alloc_epoch() { return g_alloc_epoch.load(std::memory_order_relaxed); }
+
 fss_status DevBuf::ensure(size_t nbytes)
 {
     if (nbytes <= bytes && p) return FSS_OK;
     release();
+    g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
     if (nbytes == 0) nbytes = 16;
     hipError_t e = hipMalloc(&p, nbytes);
     if (e != hipSuccess) {
@@ -35,7 +41,10 @@ fss_status DevBuf::ensure(size_t nbytes)
 
 void DevBuf::release()
 {
-    if (p) (void)hipFree(p);
+    if (p) {
+        (void)hipFree(p);
+        g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
+    }
     p = nullptr;
     bytes = 0;
 }

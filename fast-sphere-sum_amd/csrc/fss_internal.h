@@ -34,6 +34,11 @@ void set_error(const std::string &msg);
         if (s_ != FSS_OK) return s_;                                                    \
     } while (0)
 
+// Count of device-buffer (re)allocations and frees in this process.  A
+// captured graph holds raw workspace addresses: it is valid only while this
+// count is unchanged since its capture.
+uint64_t alloc_epoch();
+
 // Growable device buffer (move-only, freed on destruction).
 struct DevBuf {
     void *p = nullptr;
@@ -275,6 +280,8 @@ struct fss_ctx {
     int64_t graph_n = 0;
     double graph_dt = 0;
     int graph_tree = -1;
+    int graph_near_r = -1;
+    uint64_t graph_epoch = 0; // alloc_epoch() when the graph was captured
     bool capturing = false; // inside hipStreamBeginCapture: no event records
 };
 

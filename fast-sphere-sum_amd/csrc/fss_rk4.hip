@@ -129,8 +129,12 @@ fss_status rk4_run_graph(fss_ctx *ctx, double *x, double *y, double *z, double *
 {
     if (steps <= 0) return FSS_OK;
     const void *key[6] = {x, y, z, vort, area, nullptr};
+    // the graph holds the workspace addresses and the plan's launch shapes of
+    // its capture: any reallocation since then (a larger direct sum on this
+    // context, a rebuilt tree, another near tile) means capturing again
     bool same = ctx->graph_exec && ctx->graph_n == n && ctx->graph_dt == dt &&
-                ctx->graph_tree == use_tree;
+                ctx->graph_tree == use_tree && ctx->graph_near_r == ctx->near_r &&
+                ctx->graph_epoch == alloc_epoch();
     for (int i = 0; i < 5 && same; i++) same = ctx->graph_key[i] == key[i];
     int done = 0;
     if (!same) {
@@ -162,6 +166,8 @@ fss_status rk4_run_graph(fss_ctx *ctx, double *x, double *y, double *z, double *
         ctx->graph_n = n;
         ctx->graph_dt = dt;
         ctx->graph_tree = use_tree;
+        ctx->graph_near_r = ctx->near_r;
+        ctx->graph_epoch = alloc_epoch();
     }
     for (int i = done; i < steps; i++) FSS_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
     return FSS_OK;

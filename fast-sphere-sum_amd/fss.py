@@ -157,6 +157,20 @@ def _dev_ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+def _check_state(device, *arrays):
+    """Particle state arrays: float64, one length, all on the context's device.
+    The kernels read n doubles from every pointer, so a shorter or narrower
+    array would be read past its end."""
+    import torch
+    n = arrays[0].numel()
+    for t in arrays:
+        if not (isinstance(t, torch.Tensor) and t.dtype == torch.float64 and t.numel() == n
+                and t.device == device and t.is_contiguous()):
+            raise FssError(1, f"particle arrays must be contiguous float64 tensors of one length on "
+                              f"{device} (got {getattr(t, 'dtype', type(t))}, {getattr(t, 'numel', lambda: '?')()} "
+                              f"elements on {getattr(t, 'device', '?')}, expected {n})")
+
+
 # ---------------- grids and fields (host) ----------------
 
 def grid_icosahedral(level):
@@ -305,6 +319,7 @@ class Context:
         _check(lib().fss_sync(self.h))
 
     def _particles(self, x, y, z, vort, area):
+        _check_state(self.device, x, y, z, vort, area)
         return Particles(_dev_ptr(x).value, _dev_ptr(y).value, _dev_ptr(z).value,
                          _dev_ptr(vort).value, _dev_ptr(area).value, x.numel())
 
@@ -317,6 +332,7 @@ class Context:
         return out, Velocity(*(t.data_ptr() for t in out))
 
     def tree_build(self, x, y, z):
+        _check_state(self.device, x, y, z)
         _check(lib().fss_tree_build(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), x.numel()))
         self.n = x.numel()
 
@@ -325,6 +341,7 @@ class Context:
         concatenated in x, y, z; later calls act on the concatenated state."""
         n = np.ascontiguousarray(sizes, dtype=np.int64)
         assert int(n.sum()) == x.numel()
+        _check_state(self.device, x, y, z)
         lv = None if tree_levels is None else np.ascontiguousarray(tree_levels, dtype=np.int32)
         _check(lib().fss_batch_build(self.h, len(n), _np_ptr(n), None if lv is None else _np_ptr(lv),
                                      _dev_ptr(x), _dev_ptr(y), _dev_ptr(z)))
@@ -392,10 +409,12 @@ class Context:
         return tuple(o)
 
     def rk4_step(self, x, y, z, vort, area, dt, use_tree=False):
+        _check_state(self.device, x, y, z, vort, area)
         _check(lib().fss_rk4_step(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), _dev_ptr(vort),
                                   _dev_ptr(area), x.numel(), dt, int(use_tree)))
 
     def rk4_run_graph(self, x, y, z, vort, area, dt, steps, use_tree=False):
+        _check_state(self.device, x, y, z, vort, area)
         _check(lib().fss_rk4_run_graph(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), _dev_ptr(vort),
                                        _dev_ptr(area), x.numel(), dt, int(use_tree), steps))
 
@@ -417,6 +436,7 @@ class Context:
         return vort, tr, to, nf.value
 
     def project_points(self, x, y, z, vort):
+        _check_state(self.device, x, y, z, vort)
         _check(lib().fss_project_points(self.h, _dev_ptr(x), _dev_ptr(y), _dev_ptr(z), _dev_ptr(vort),
                                         x.numel()))
 

@@ -1,0 +1,180 @@
+"""GPU parity on the benchmark workloads' own shapes, against the CPU oracle.
+
+* Level-5 icosahedral grid (N = 10,242), Rossby-Haurwitz field: the FP64
+  direct sum at every particle, and ten RK4 steps at dt = 0.01 on the direct
+  path (rhs_eval.h:15-32, new_dynamics.cpp:142-238 intent).
+* Batched uneven instances (the 64-instance workload's construction: uniform
+  random points, equal areas 4 pi / N, 1-6 Gaussian vortices, N drawn
+  log-uniform, here in [2e4, 1e5]): every instance of one batched launch
+  against the oracle's BLTC tree on that instance alone, in FP64 and mixed
+  precision, and two batched RK4 steps replayed from the hipGraph against the
+  oracle's RK4 with the same frozen tree.
+* Mixed-precision tree stages against the oracle directly.
+
+Bars: FP64 rel-L2 <= 1e-12; mixed precision rel-L2 <= 1e-6."""
+import numpy as np
+import pytest
+
+from conftest import icos_case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12
+TOL_MIXED = 1e-6
+
+
+def _dev(*arrs):
+    import torch
+    return [torch.from_numpy(np.ascontiguousarray(a)).to("cuda") for a in arrs]
+
+
+def _host(*ts):
+    return [t.cpu().numpy() for t in ts]
+
+
+def _vel_err(got, ref):
+    U = np.stack(got[:3], 1)
+    R = np.stack(ref[:3], 1)
+    return rel_l2(U, R), rel_l2(got[3], ref[3])
+
+
+@pytest.fixture(scope="module")
+def level5(fsslib):
+    return icos_case(fsslib, 5)
+
+
+def test_level5_direct_fp64_all_targets(fsslib, orc, level5):
+    x, y, z, vort, area, _ = level5
+    print(f"direct fp64 N={len(x)} (all targets)", flush=True)
+    ctx = fsslib.Context(precision=fsslib.FP64)
+    got = _host(*ctx.velocity_direct(*_dev(x, y, z, vort, area)))
+    ref = orc.rhs_direct(x, y, z, vort, area)
+    eu, ew = _vel_err(got, ref)
+    print("rel-L2 u", eu, "dvort", ew)
+    assert eu <= TOL_FP64 and ew <= TOL_FP64
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_level5_rk4_ten_steps_direct(fsslib, orc, level5, graph):
+    """Ten RK4 steps, dt = 0.01, direct sum, projection after every stage:
+    the device state after the tenth step against the oracle's."""
+    x, y, z, vort, area, _ = level5
+    dt, steps = 0.01, 10
+    print(f"rk4 direct N={len(x)} steps={steps} dt={dt} graph={graph}", flush=True)
+    ctx = fsslib.Context(precision=fsslib.FP64)
+    d = _dev(x, y, z, vort, area)
+    if graph:
+        ctx.rk4_run_graph(*d, dt, steps, use_tree=False)
+    else:
+        for _ in range(steps):
+            ctx.rk4_step(*d, dt, use_tree=False)
+    got = _host(*d[:4])
+    ref = (x, y, z, vort)
+    for _ in range(steps):
+        ref = orc.rk4_step(*ref, area, dt)
+    errs = [rel_l2(g, r) for g, r in zip(got, ref)]
+    moved = rel_l2(np.stack(ref[:3], 1), np.stack((x, y, z), 1))
+    print("rel-L2 x, y, z, vort", errs, "displacement", moved)
+    assert moved > 1e-3  # the state did move
+    assert max(errs) <= TOL_FP64
+
+
+def _batch_instances(fss, K, seed, nmin=2e4, nmax=1e5):
+    """The 64-instance workload's construction at smaller sizes."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), K)).astype(np.int64)
+    nvort = rng.integers(1, 7, K)
+    parts = []
+    for k in range(K):
+        x, y, z = fss.grid_random(int(sizes[k]), seed + 1 + k)
+        a = np.full(len(x), 4 * np.pi / len(x))
+        v = fss.gauss_vortices(x, y, z, a, int(nvort[k]), 2.0, 0.05, 0.3, seed=seed + 100 + k)
+        parts.append((x, y, z, v, a))
+    return sizes, parts
+
+
+@pytest.fixture(scope="module")
+def batch4(fsslib):
+    sizes, parts = _batch_instances(fsslib, 4, 4044)
+    levels = [fsslib.levels_for(int(n), 21.0) for n in sizes]
+    print(f"batch sizes={sizes.tolist()} levels={levels}", flush=True)
+    return sizes, parts, levels
+
+
+def _oracle_trees(orc, parts, levels):
+    return [orc.Tree(p[0], p[1], p[2], L, theta=0.7, many_count=10, degree=2, assign=orc.ASSIGN_ROBUST,
+                     eval_mode=orc.EVAL_BLTC, node_rule=1) for p, L in zip(parts, levels)]
+
+
+@pytest.fixture(scope="module")
+def batch4_oracle(orc, batch4):
+    sizes, parts, levels = batch4
+    trees = _oracle_trees(orc, parts, levels)
+    refs = [T.rhs(p[3], p[4], nthreads=16) for T, p in zip(trees, parts)]
+    return trees, refs
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_instances_match_oracle(fsslib, batch4, batch4_oracle, precision):
+    sizes, parts, levels = batch4
+    trees, refs = batch4_oracle
+    cat = _dev(*[np.concatenate([p[c] for p in parts]) for c in range(5)])
+    print(f"batched velocity precision={precision} N total={int(sizes.sum())}", flush=True)
+    ctx = fsslib.Context(precision=precision, eval_mode=fsslib.EVAL_BLTC, assign_mode=fsslib.ASSIGN_ROBUST,
+                         node_rule=fsslib.NODES_SIN2, degree=2, theta=0.7)
+    ctx.batch_build(*cat[:3], sizes, levels)
+    got = _host(*ctx.velocity_tree(*cat))
+    tol = TOL_FP64 if precision == 0 else TOL_MIXED
+    off = 0
+    for k, n in enumerate(sizes):
+        s = slice(off, off + int(n))
+        eu, ew = _vel_err([g[s] for g in got], refs[k])
+        print(f"instance {k} N={n} L={levels[k]}: rel-L2 u {eu:.3e} dvort {ew:.3e}")
+        assert eu <= tol and ew <= tol, (k, eu, ew)
+        off += int(n)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_rk4_graph_matches_oracle(fsslib, orc, batch4, batch4_oracle, precision):
+    """Two RK4 steps of the batch replayed from the captured hipGraph: every
+    instance against two oracle RK4 steps with its frozen t = 0 tree."""
+    sizes, parts, levels = batch4
+    trees, _ = batch4_oracle
+    dt = 0.005
+    cat = _dev(*[np.concatenate([p[c] for p in parts]) for c in range(5)])
+    print(f"batched rk4 graph precision={precision}", flush=True)
+    ctx = fsslib.Context(precision=precision, eval_mode=fsslib.EVAL_BLTC, assign_mode=fsslib.ASSIGN_ROBUST,
+                         node_rule=fsslib.NODES_SIN2, degree=2, theta=0.7)
+    ctx.batch_build(*cat[:3], sizes, levels)
+    ctx.rk4_run_graph(*cat, dt, 2, use_tree=True)
+    got = _host(*cat[:4])
+    tol = TOL_FP64 if precision == 0 else TOL_MIXED
+    off = 0
+    for k, n in enumerate(sizes):
+        s = slice(off, off + int(n))
+        x, y, z, v, a = parts[k]
+        ref = (x, y, z, v)
+        for _ in range(2):
+            ref = orc.rk4_step(*ref, a, dt, tree=trees[k], nthreads=16)
+        # the change over the two steps is what the velocities decide
+        errs = [rel_l2(g[s] - p0, r - p0) for g, r, p0 in zip(got, ref, (x, y, z, v))]
+        print(f"instance {k}: rel-L2 of the step increments x, y, z, vort {errs}")
+        assert max(errs) <= tol, (k, errs)
+        off += int(n)
+
+
+@pytest.mark.parametrize("evalm,rule", [(0, 0), (1, 1)])
+def test_tree_mixed_against_oracle(fsslib, orc, level5, evalm, rule):
+    """The mixed-precision treecode against the oracle's FP64 tree on the
+    same list (reference dispatch and BLTC)."""
+    x, y, z, vort, area, _ = level5
+    print(f"tree mixed N={len(x)} eval={evalm}", flush=True)
+    c32 = fsslib.Context(tree_levels=5, assign_mode=1, eval_mode=evalm, node_rule=rule, precision=fsslib.MIXED)
+    d = _dev(x, y, z, vort, area)
+    c32.tree_build(*d[:3])
+    got = _host(*c32.velocity_tree(*d))
+    T = orc.Tree(x, y, z, 5, theta=0.7, many_count=10, degree=2, assign=1, eval_mode=evalm, node_rule=rule)
+    ref = T.rhs(vort, area, nthreads=16)
+    eu, ew = _vel_err(got, ref)
+    print("mixed vs oracle", eu, ew)
+    assert eu <= TOL_MIXED and ew <= TOL_MIXED

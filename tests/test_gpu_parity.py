@@ -165,6 +165,26 @@ def test_rk4_graph_replays_match_eager(fsslib, case4):
         np.testing.assert_array_equal(g, r)
 
 
+def test_rk4_graph_recaptured_after_workspace_grows(fsslib, case4, case5):
+    """A captured graph holds the context's workspace addresses: a larger
+    direct sum on the same context reallocates that workspace, and the next
+    replay must capture again instead of using the freed buffers."""
+    x, y, z, vort, area, _ = case4
+    dt = 0.01
+    ctx = fsslib.Context(precision=fsslib.FP64)
+    a = _dev(x, y, z, vort, area)
+    b = _dev(x, y, z, vort, area)
+    ctx.rk4_run_graph(*b, dt, 1)
+    big = _dev(*case5[:5])
+    ctx.velocity_direct(*big)  # grows the record workspace past the captured size
+    ctx.rk4_run_graph(*b, dt, 2)
+    ref = fsslib.Context(precision=fsslib.FP64)
+    for _ in range(3):
+        ref.rk4_step(*a, dt)
+    for g, r in zip(_host(*b[:4]), _host(*a[:4])):
+        np.testing.assert_array_equal(g, r)
+
+
 @pytest.fixture(scope="module")
 def random_case(fsslib):
     n = 12000
