@@ -392,7 +392,7 @@ __global__ void k_plan_rcount(const int32_t *__restrict__ inter, const int32_t *
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
-    const int32_t *o = inter + 7 * sorted_idx[k];
+    const int32_t *o = inter + 7 * (int64_t)sorted_idx[k];
     int type = o[6];
     int64_t gs = level_offset(o[1]) + o[3];
     bool proxy_src = bltc && (type == 1 || type == 3);
@@ -409,7 +409,7 @@ __global__ void k_plan_ranges(const int32_t *__restrict__ inter, const int32_t *
     // (summed over the wave first: one atomic per wave and class)
     unsigned long long accA = 0, accB = 0;
     if (k < m) {
-        const int32_t *o = inter + 7 * sorted_idx[k];
+        const int32_t *o = inter + 7 * (int64_t)sorted_idx[k];
         int type = o[6];
         bool proxy_src = bltc && (type == 1 || type == 3);
         bool clsA = (type == 0 || type == 1);
@@ -426,7 +426,7 @@ __global__ void k_plan_ranges(const int32_t *__restrict__ inter, const int32_t *
         if (accB) atomicAdd(&pair_acc[1], accB);
     }
     if (k >= m) return;
-    const int32_t *o = inter + 7 * sorted_idx[k];
+    const int32_t *o = inter + 7 * (int64_t)sorted_idx[k];
     int type = o[6];
     int64_t gs = level_offset(o[1]) + o[3];
     bool proxy_src = bltc && (type == 1 || type == 3);
@@ -894,6 +894,14 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
         gen++;
     }
     FSS_HIP(hipStreamSynchronize(s));
+    // the plan indexes interactions and their source ranges (at most two per
+    // interaction) with int32
+    if (ninter > INT32_MAX / 2) {
+        set_error("tree_build: " + std::to_string(ninter) +
+                  " interactions exceed the plan's 32-bit indices (at most " + std::to_string(INT32_MAX / 2) +
+                  "); use fewer tree levels or a larger theta");
+        return FSS_ERR_INVALID;
+    }
     T.ngen = gen;
     T.ninter = ninter;
     {
