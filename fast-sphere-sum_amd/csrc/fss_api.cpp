@@ -606,4 +606,23 @@ fss_status fss_timing_mean(fss_ctx *ctx, int64_t *calls, double *stage_ms, doubl
     return FSS_OK;
 }
 
+fss_status fss_timing_far_mean(fss_ctx *ctx, double *far_ms)
+{
+    if (!ctx || !far_ms) return set_error("null argument"), FSS_ERR_INVALID;
+    Timing &t = ctx->timing;
+    if (!t.valid || t.calls < 1) return set_error("no timed call yet"), FSS_ERR_STATE;
+    if (t.calls > Timing::RING)
+        return set_error("more timed calls than event slots; reset more often"), FSS_ERR_STATE;
+    double sf = 0;
+    for (int64_t i = 0; i < t.calls; i++) {
+        hipEvent_t *ev = t.ev[i];
+        FSS_HIP(hipEventSynchronize(ev[3]));
+        float f = 0;
+        FSS_HIP(hipEventElapsedTime(&f, ev[4], ev[1]));
+        sf += f;
+    }
+    *far_ms = sf / t.calls;
+    return FSS_OK;
+}
+
 } // extern "C"

@@ -183,14 +183,20 @@ fss_status velocity_direct(fss_ctx *ctx, const fss_particles *p, const int64_t *
         FSS_TRY(ctx->src64.ensure(sizeof(double4) * n));
         hipLaunchKernelGGL(k_pack64, dim3(gp), dim3(256), 0, s, p->x, p->y, p->z, p->vort, p->area, n,
                            ctx->src64.as<double4>());
-        if (!ctx->capturing) FSS_HIP(hipEventRecord(ev[1], s));
+        if (!ctx->capturing) {
+            FSS_HIP(hipEventRecord(ev[4], s));
+            FSS_HIP(hipEventRecord(ev[1], s));
+        }
         hipLaunchKernelGGL(k_direct64, dim3(gt), dim3(DTILE), 0, s, ctx->src64.as<double4>(), n, idx,
                            m, scale, ctx->cfg.omega, out->u, out->v, out->w, out->dvort);
     } else {
         FSS_TRY(ctx->src32.ensure(sizeof(float4) * n));
         hipLaunchKernelGGL(k_pack32, dim3(gp), dim3(256), 0, s, p->x, p->y, p->z, p->vort, p->area, n,
                            ctx->src32.as<float4>());
-        if (!ctx->capturing) FSS_HIP(hipEventRecord(ev[1], s));
+        if (!ctx->capturing) {
+            FSS_HIP(hipEventRecord(ev[4], s));
+            FSS_HIP(hipEventRecord(ev[1], s));
+        }
         hipLaunchKernelGGL(k_direct32, dim3(gt), dim3(DTILE), 0, s, ctx->src32.as<float4>(), n, idx,
                            m, scale, ctx->cfg.omega, out->u, out->v, out->w, out->dvort);
     }

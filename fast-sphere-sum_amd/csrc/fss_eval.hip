@@ -344,14 +344,18 @@ __global__ void __launch_bounds__(256) k_upward_pair(const int4 *__restrict__ it
 #pragma unroll
     for (int k = 0; k < P; k++)
         for (int off = 16; off > 0; off >>= 1) M[k] += __shfl_xor(M[k], off, 64);
-    if (!mine || hl >= P) return;
-    if (it.w < 0) {
-        write_charge<Real, D>(M, vinv, p0, it.x, hl, src);
-    } else {
-        double mk = M[0];
+    if (!mine) return;
+    // 32 lanes per item: lane hl writes charges (or partial moments) hl and,
+    // for P > 32 (degree 7 and 8), hl + 32
+    for (int kk = hl; kk < P; kk += 32) {
+        if (it.w < 0) {
+            write_charge<Real, D>(M, vinv, p0, it.x, kk, src);
+        } else {
+            double mk = M[0];
 #pragma unroll
-        for (int k = 1; k < P; k++) mk = hl == k ? M[k] : mk;
-        partial[(int64_t)it.w * P + hl] = mk;
+            for (int k = 1; k < P; k++) mk = kk == k ? M[k] : mk;
+            partial[(int64_t)it.w * P + kk] = mk;
+        }
     }
 }
 
@@ -1089,6 +1093,7 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
         hipLaunchKernelGGL(k_gather64, dim3(gs), dim3(256), 0, s, pp->x, pp->y, pp->z, pp->vort,
                            pp->area, T.recid.as<int32_t>(), p0, T.proxy_xyz.as<double4>(), nprox, src);
         if (bltc) FSS_TRY((upward<double, D>(ctx, src)));
+        if (ev) FSS_HIP(hipEventRecord(ev[4], s));
         if (T.n_proxy > 0)
         {
             hipLaunchKernelGGL((k_proxy_list<double, D>), dim3((unsigned)div_up(T.n_proxy, 4)), dim3(256), 0, s,
@@ -1115,6 +1120,7 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
                            T.recnode.as<int32_t>(), T.tri_info.as<double>(), bltc ? 0 : 1, src,
                            aoff);
         if (bltc) FSS_TRY((upward<float, D>(ctx, src)));
+        if (ev) FSS_HIP(hipEventRecord(ev[4], s));
         if (T.n_proxy > 0)
         {
             hipLaunchKernelGGL((k_proxy_list<float, D>), dim3((unsigned)div_up(T.n_proxy, 4)), dim3(256), 0, s,

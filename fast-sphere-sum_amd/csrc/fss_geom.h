@@ -16,6 +16,8 @@
 #include <cmath>
 #include <cstdint>
 
+#include "fss_acos_cr.h"
+
 #define FSS_HD __host__ __device__ inline
 
 namespace fssg {
@@ -99,6 +101,20 @@ FSS_HD double great_circ_dist(const double *p1, const double *p2, double radius)
     if (cl < -1.0) cl = -1.0;
     if (1.0 < cl) cl = 1.0;
     return acos(cl) * radius;
+}
+
+// The same distance with the correctly rounded arccosine: decides the MAC of
+// pairs whose separation lies within FSS_MAC_TIE of theta, where the last bit
+// of acos could decide it (any two libms that round acos to within a few ulp
+// change a separation by less than 1e-15).
+#define FSS_MAC_TIE 1e-12
+FSS_HD double great_circ_dist_cr(const double *p1, const double *p2, double radius)
+{
+    double s = dot3(p1, p2) / (norm3(p1) * norm3(p2));
+    double cl = s;
+    if (cl < -1.0) cl = -1.0;
+    if (1.0 < cl) cl = 1.0;
+    return fsscr::acos_cr(cl) * radius;
 }
 
 // Solve [p1 p2 p3] c = p (columns p1, p2, p3) as LAPACK dgesv does for n = 3

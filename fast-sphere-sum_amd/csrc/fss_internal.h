@@ -211,7 +211,8 @@ struct Tree {
 };
 
 // HIP events around each velocity call: a ring of RING slots, each holding
-// {stage start, main kernel start, main kernel end, stage end}.  Recording
+// {stage start, main kernel start, main kernel end, stage end, far kernel
+// start}; the far kernel ends where the main kernel starts.  Recording
 // never synchronizes; the mean over the slots since the last reset is read
 // with fss_timing_mean().
 // SAL state: series coefficients c_n (n = 0..nmax, c_0 = 0), the tabulated
@@ -224,13 +225,14 @@ struct Sal {
     double mu = 0, sigma = 1; // normalisation of the learned remainder
     double s_min = 1e-3;      // the network input maps log s from [s_min, 1] onto [-1, 1]
     DevBuf c, tab, P, Pm, Pv, w;
+    DevBuf w2op; // the hidden layer's weights in matrix-core operand order (fp16 hi/lo)
     bool has_mlp = false;
     int64_t adam_t = 0;
 };
 
 struct Timing {
     static constexpr int RING = 256;
-    hipEvent_t ev[RING][4] = {};
+    hipEvent_t ev[RING][5] = {}; // + [4]: start of the far-field (proxy) kernel
     int64_t calls = 0;  // calls recorded since the last reset
     int cur = -1;       // slot of the current call
     bool valid = false;
@@ -342,6 +344,8 @@ fss_status sal_mlp_params(fss_ctx *ctx, float *host, int set);
 fss_status sal_mlp_init(fss_ctx *ctx, uint64_t seed);
 fss_status sal_mlp_train(fss_ctx *ctx, int steps, int64_t nsamp, double lr, double s_min,
                          uint64_t seed, double *final_loss);
+// matrix-core operand table of the current network parameters
+fss_status sal_mlp_prepare(fss_ctx *ctx);
 fss_status sal_mlp_grad(fss_ctx *ctx, const float *x_host, const float *y_host, int64_t m,
                         float *grad_host, float *loss_host);
 fss_status sal_direct(fss_ctx *ctx, const double *x, const double *y, const double *z,

@@ -9,6 +9,8 @@
 // proj = project_points (helpers.h:362-372): x <- x/|x| and
 // omega += -2 Omega (z_new - z_old), which keeps omega + 2 Omega z per particle.
 // The tree is the t = 0 tree (new_dynamics.cpp:84-88 builds it once).
+// Compiled without FMA contraction: every product and sum of the update is
+// rounded separately, as the reference's scalar_mult and vec_add round them.
 #include <cmath>
 
 #include "fss_internal.h"

@@ -103,16 +103,38 @@ __global__ void k_adam(float *P, const float *__restrict__ g, float *mom, float 
     P[q] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
 }
 
-__global__ void k_mlp_eval(const float *__restrict__ P, const double *__restrict__ s, int64_t m,
-                           double A, double mu, double sigma, float log_smin, float inv_span,
-                           double *out)
+// the hidden layer's weights W2[j][k] (row-major in P) in the operand order
+// of mlp_wave64, split into fp16 hi and lo
+__global__ void k_mlp_w2op(const float *__restrict__ P, half4v *w2op)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    float h1[H], h2[H];
-    double si = s[i];
-    float y = mlp_forward(P, mlp_input(si, log_smin, inv_span), h1, h2);
-    out[i] = A * (0.5 / si - 1.0) + mu + sigma * (double)y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= MLP_W2OP) return;
+    const int lane = e & 63, part = (e >> 6) & 1, r = (e >> 7) & 1, sl = e >> 8;
+    const int j = 32 * r + (lane & 31);
+    half4v o;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int k = 8 * sl + 4 * (lane >> 5) + i;
+        _Float16 hi, lo;
+        split_f16(P[OFF_W2 + j * H + k], hi, lo);
+        o[i] = part ? lo : hi;
+    }
+    w2op[e] = o;
+}
+
+// the learned kernel at m separations s, 64 per wave on the matrix cores
+// (lanes past m evaluate a dummy input)
+__global__ void __launch_bounds__(256) k_mlp_eval(const float *__restrict__ P, const half4v *__restrict__ w2op,
+                                                  const double *__restrict__ s, int64_t m, double A, double mu,
+                                                  double sigma, float log_smin, float inv_span, double *out)
+{
+    __shared__ MlpLds M;
+    mlp_stage(M, P, w2op);
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double si = i < m ? s[i] : 1.0;
+    const float y = mlp_wave64(M, mlp_input(si, log_smin, inv_span));
+    if (i < m) out[i] = A * (0.5 / si - 1.0) + mu + sigma * (double)y;
 }
 
 // training inputs (FP32) and targets (G - singular - mu) / sigma
@@ -132,21 +154,15 @@ __global__ void k_mlp_targets(const double *__restrict__ s, const double *__rest
 // ------------------------------------------------------------------
 constexpr int STILE = 256;
 
-template <int KIND>
-__global__ void __launch_bounds__(STILE) k_sal_direct(const double *__restrict__ x,
-                                                      const double *__restrict__ y,
-                                                      const double *__restrict__ z,
-                                                      const double *__restrict__ w, int64_t n,
-                                                      const int64_t *__restrict__ idx, int64_t m,
-                                                      const double *__restrict__ tab, int64_t M,
-                                                      const float *__restrict__ P, double A,
-                                                      double mu, double sigma, double g0,
-                                                      float log_smin, float inv_span, double *out)
+__global__ void __launch_bounds__(STILE) k_sal_direct_table(const double *__restrict__ x,
+                                                             const double *__restrict__ y,
+                                                             const double *__restrict__ z,
+                                                             const double *__restrict__ w, int64_t n,
+                                                             const int64_t *__restrict__ idx, int64_t m,
+                                                             const double *__restrict__ tab, int64_t M,
+                                                             double g0, double *out)
 {
     __shared__ double4 tile[STILE];
-    __shared__ float Ps[NPARAM];
-    if (KIND == 1)
-        for (int q = threadIdx.x; q < NPARAM; q += STILE) Ps[q] = P[q];
     int64_t r = (int64_t)blockIdx.x * STILE + threadIdx.x;
     int64_t i = r < m ? (idx ? idx[r] : r) : 0;
     double xi = x[i], yi = y[i], zi = z[i];
@@ -161,16 +177,47 @@ __global__ void __launch_bounds__(STILE) k_sal_direct(const double *__restrict__
             double4 sj = tile[k];
             double dx = sj.x - xi, dy = sj.y - yi, dz = sj.z - zi;
             double s = 0.5 * sqrt(dx * dx + dy * dy + dz * dz);
-            double gk;
-            if (base + k == i) {
-                gk = g0; // self term: the truncated series at theta = 0
-            } else if (KIND == 0) {
-                gk = table_eval(tab, M, s);
-            } else {
-                float h1[H], h2[H];
-                float yv = mlp_forward(Ps, mlp_input(s, log_smin, inv_span), h1, h2);
-                gk = A * (0.5 / s - 1.0) + mu + sigma * (double)yv;
-            }
+            // self term: the truncated series at theta = 0
+            double gk = base + k == i ? g0 : table_eval(tab, M, s);
+            acc = fma(gk, sj.w, acc);
+        }
+    }
+    if (r < m) out[r] = acc;
+}
+
+// the same with the learned kernel: every source of a tile is evaluated for
+// the 64 targets of a wave at once on the matrix cores (the loop is uniform
+// over the block, so every wave is whole at each network evaluation)
+__global__ void __launch_bounds__(STILE) k_sal_direct_mlp(const double *__restrict__ x,
+                                                          const double *__restrict__ y,
+                                                          const double *__restrict__ z,
+                                                          const double *__restrict__ w, int64_t n,
+                                                          const int64_t *__restrict__ idx, int64_t m,
+                                                          const float *__restrict__ P,
+                                                          const half4v *__restrict__ w2op, double A,
+                                                          double mu, double sigma, double g0,
+                                                          float log_smin, float inv_span, double *out)
+{
+    __shared__ double4 tile[STILE];
+    __shared__ MlpLds Ml;
+    mlp_stage(Ml, P, w2op);
+    int64_t r = (int64_t)blockIdx.x * STILE + threadIdx.x;
+    int64_t i = r < m ? (idx ? idx[r] : r) : 0;
+    double xi = x[i], yi = y[i], zi = z[i];
+    double acc = 0.0;
+    for (int64_t base = 0; base < n; base += STILE) {
+        __syncthreads();
+        int64_t j = base + threadIdx.x;
+        if (j < n) tile[threadIdx.x] = make_double4(x[j], y[j], z[j], w[j]);
+        __syncthreads();
+        int cnt = (int)min((int64_t)STILE, n - base);
+        for (int k = 0; k < cnt; k++) {
+            double4 sj = tile[k];
+            double dx = sj.x - xi, dy = sj.y - yi, dz = sj.z - zi;
+            double s = 0.5 * sqrt(dx * dx + dy * dy + dz * dz);
+            const bool self = base + k == i;
+            const float yv = mlp_wave64(Ml, mlp_input(self ? 1.0 : s, log_smin, inv_span));
+            double gk = self ? g0 : A * (0.5 / s - 1.0) + mu + sigma * (double)yv;
             acc = fma(gk, sj.w, acc);
         }
     }
@@ -361,8 +408,10 @@ fss_status sal_kernel_eval(fss_ctx *ctx, int kind, const double *s_dev, int64_t 
     hipStream_t s = ctx->stream;
     if (kind == 0) return sal_series_eval(ctx, s_dev, m, g_dev);
     if (!S.has_mlp) return set_error("sal: no network"), FSS_ERR_STATE;
-    hipLaunchKernelGGL(k_mlp_eval, dim3((unsigned)div_up(m, 256)), dim3(256), 0, s, S.P.as<float>(), s_dev,
-                       m, S.A, S.mu, S.sigma, (float)std::log(S.s_min), (float)(1.0 / -std::log(S.s_min)), g_dev);
+    FSS_TRY(sal_mlp_prepare(ctx));
+    hipLaunchKernelGGL(k_mlp_eval, dim3((unsigned)div_up(m, 256)), dim3(256), 0, s, S.P.as<float>(),
+                       S.w2op.as<half4v>(), s_dev, m, S.A, S.mu, S.sigma, (float)std::log(S.s_min),
+                       (float)(1.0 / -std::log(S.s_min)), g_dev);
     FSS_HIP(hipGetLastError());
     return FSS_OK;
 }
@@ -374,27 +423,40 @@ fss_status sal_direct(fss_ctx *ctx, const double *x, const double *y, const doub
     hipStream_t s = ctx->stream;
     if (!S.c.p) return set_error("sal: series not set"), FSS_ERR_STATE;
     if (kind == 1 && !S.has_mlp) return set_error("sal: no network"), FSS_ERR_STATE;
+    if (kind == 1) FSS_TRY(sal_mlp_prepare(ctx));
     if (!idx) m = n;
     FSS_TRY(S.w.ensure(sizeof(double) * n));
     hipLaunchKernelGGL(k_mul, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, eta, area, n, S.w.as<double>());
     if (!ctx->capturing) {
         timing_begin(ctx->timing);
         FSS_HIP(hipEventRecord(ctx->timing.slot()[0], s));
+        FSS_HIP(hipEventRecord(ctx->timing.slot()[4], s));
         FSS_HIP(hipEventRecord(ctx->timing.slot()[1], s));
     }
     unsigned g = (unsigned)div_up(m, STILE);
     float lmin = (float)std::log(S.s_min), ispan = (float)(1.0 / -std::log(S.s_min));
     if (kind == 0)
-        hipLaunchKernelGGL((k_sal_direct<0>), dim3(g), dim3(STILE), 0, s, x, y, z, S.w.as<double>(), n, idx, m,
-                           S.tab.as<double>(), S.M, (const float *)nullptr, S.A, S.mu, S.sigma, S.g0, lmin, ispan, out);
+        hipLaunchKernelGGL(k_sal_direct_table, dim3(g), dim3(STILE), 0, s, x, y, z, S.w.as<double>(), n, idx, m,
+                           S.tab.as<double>(), S.M, S.g0, out);
     else
-        hipLaunchKernelGGL((k_sal_direct<1>), dim3(g), dim3(STILE), 0, s, x, y, z, S.w.as<double>(), n, idx, m,
-                           S.tab.as<double>(), S.M, S.P.as<float>(), S.A, S.mu, S.sigma, S.g0, lmin, ispan, out);
+        hipLaunchKernelGGL(k_sal_direct_mlp, dim3(g), dim3(STILE), 0, s, x, y, z, S.w.as<double>(), n, idx, m,
+                           S.P.as<float>(), S.w2op.as<half4v>(), S.A, S.mu, S.sigma, S.g0, lmin, ispan, out);
     if (!ctx->capturing) {
         FSS_HIP(hipEventRecord(ctx->timing.slot()[2], s));
         FSS_HIP(hipEventRecord(ctx->timing.slot()[3], s));
         ctx->timing.valid = true;
     }
+    FSS_HIP(hipGetLastError());
+    return FSS_OK;
+}
+
+fss_status sal_mlp_prepare(fss_ctx *ctx)
+{
+    Sal &S = ctx->sal;
+    if (!S.has_mlp) return set_error("sal: no network"), FSS_ERR_STATE;
+    FSS_TRY(S.w2op.ensure(sizeof(half4v) * MLP_W2OP));
+    hipLaunchKernelGGL(k_mlp_w2op, dim3((unsigned)div_up(MLP_W2OP, 256)), dim3(256), 0, ctx->stream,
+                       S.P.as<float>(), S.w2op.as<half4v>());
     FSS_HIP(hipGetLastError());
     return FSS_OK;
 }

@@ -49,16 +49,25 @@ __device__ inline double table_eval(const double *__restrict__ tab, int64_t M, d
 // ------------------------------------------------------------------
 // MLP
 // ------------------------------------------------------------------
+// tanh(x) = sign(x) (1 - e) / (1 + e), e = exp(-2 |x|): one v_exp_f32 and one
+// v_rcp_f32; absolute error about 1e-7 (the network's inputs and outputs are
+// FP32 quantities of order one)
+__device__ __forceinline__ float tanh_fast(float x)
+{
+    const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * fabsf(x));
+    return copysignf((1.f - e) * __builtin_amdgcn_rcpf(1.f + e), x);
+}
+
 __device__ inline float mlp_forward(const float *__restrict__ P, float x, float *h1, float *h2)
 {
 #pragma unroll 8
-    for (int j = 0; j < H; j++) h1[j] = tanhf(fmaf(P[OFF_W1 + j], x, P[OFF_B1 + j]));
+    for (int j = 0; j < H; j++) h1[j] = tanh_fast(fmaf(P[OFF_W1 + j], x, P[OFF_B1 + j]));
     for (int j = 0; j < H; j++) {
         float a = P[OFF_B2 + j];
         const float *w = P + OFF_W2 + j * H;
 #pragma unroll 16
         for (int k = 0; k < H; k++) a = fmaf(w[k], h1[k], a);
-        h2[j] = tanhf(a);
+        h2[j] = tanh_fast(a);
     }
     float y = P[OFF_B3];
 #pragma unroll 16
@@ -83,6 +92,117 @@ __device__ inline double mlp_kernel(const float *__restrict__ P, double s, doubl
     float h1[H], h2[H];
     float y = mlp_forward(P, mlp_input(s, log_smin, inv_span), h1, h2);
     return A * (0.5 / s - 1.0) + mu + sigma * (double)y;
+}
+
+// ------------------------------------------------------------------
+// the network on the matrix cores: 64 inputs per wave
+// ------------------------------------------------------------------
+// The 64 x 64 hidden layer of 64 inputs is a [64 x 64] x [64 x 64] product,
+// evaluated transposed, Z2^T = W2 H1^T, by v_mfma_f32_32x32x8_f16 on 2 x 2
+// blocks of 32 x 32 and 8 K-slices of 8.  FP32 operands are split into fp16
+// hi + lo (hi = fp16(v), lo = fp16(v - hi)) and multiplied as hi.hi + hi.lo +
+// lo.hi with FP32 accumulation: about 22 significant bits per product.
+// Operand layouts of the 32x32x8 product (one wave): A[m][k] is held by lane
+// m + 32 (k / 4) as element k % 4; B[k][n] by lane n + 32 (k / 4), element
+// k % 4; the 16 results of lane l are D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32],
+// q = 0..15.  So the result rows of one block are the K rows of the next
+// product's B operand, and H1^T can be built lane by lane: lane l computes
+// the hidden units k = 8 s + 4 (l / 32) + 0..3 of inputs l % 32 and 32 + l % 32.
+typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+typedef float float16v __attribute__((ext_vector_type(16)));
+
+// W2 in A-operand order: entry ((s * 2 + r) * 2 + part) * 64 + lane holds
+// W2[32 r + lane % 32][8 s + 4 (lane / 32) + 0..3], part 0 = hi, 1 = lo
+constexpr int MLP_W2OP = 8 * 2 * 2 * 64; // half4 entries
+
+__device__ __forceinline__ void split_f16(float v, _Float16 &hi, _Float16 &lo)
+{
+    hi = (_Float16)v;
+    lo = (_Float16)(v - (float)hi);
+}
+
+// Network output for the input x of every lane of the wave (all 64 lanes
+// must be active: the matrix instructions act on the whole wave).  w1, b1, b2,
+// w3 point to the 64-entry parameter vectors, w2op to the operand table above.
+__device__ __forceinline__ float mlp_wave64(const float *__restrict__ w1, const float *__restrict__ b1,
+                                   const float *__restrict__ b2, const float *__restrict__ w3, float b3,
+                                   const half4v *__restrict__ w2op, float x)
+{
+    const int l = threadIdx.x & 63, l32 = l & 31, hh = l >> 5;
+    const float xc[2] = {__shfl(x, l32, 64), __shfl(x, 32 + l32, 64)};
+    float16v acc[2][2];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) acc[r][c][q] = 0.f;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        half4v bh[2], bl[2];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int k = 8 * s + 4 * hh + i;
+            const float wk = w1[k], bk = b1[k];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                _Float16 a, b;
+                split_f16(tanh_fast(fmaf(wk, xc[c], bk)), a, b);
+                bh[c][i] = a;
+                bl[c][i] = b;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const half4v ah = w2op[((s * 2 + r) * 2 + 0) * 64 + l];
+            const half4v al = w2op[((s * 2 + r) * 2 + 1) * 64 + l];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x8f16(ah, bh[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x8f16(ah, bl[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x8f16(al, bh[c], acc[r][c], 0, 0, 0);
+            }
+        }
+    }
+    // output layer: lane l holds hidden units j = 32 r + 8 (q / 4) + 4 hh + q % 4
+    // of inputs 32 c + l32; the other half of the units is in lane l ^ 32
+    float part[2] = {0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int j = 32 * r + 8 * (q >> 2) + 4 * hh + (q & 3);
+                part[c] = fmaf(w3[j], tanh_fast(acc[r][c][q] + b2[j]), part[c]);
+            }
+#pragma unroll
+    for (int c = 0; c < 2; c++) part[c] += __shfl_xor(part[c], 32, 64);
+    return (hh ? part[1] : part[0]) + b3; // lane l's input is column l32 of block hh
+}
+
+// Network parameters staged in LDS for mlp_wave64 by the threads of a block:
+// w1, b1, b2, w3 (64 floats each), b3, and the W2 operand table.
+struct MlpLds {
+    float w1[H], b1[H], b2[H], w3[H], b3;
+    half4v w2op[MLP_W2OP];
+};
+
+__device__ inline void mlp_stage(MlpLds &m, const float *__restrict__ P, const half4v *__restrict__ w2op)
+{
+    for (int q = threadIdx.x; q < H; q += blockDim.x) {
+        m.w1[q] = P[OFF_W1 + q];
+        m.b1[q] = P[OFF_B1 + q];
+        m.b2[q] = P[OFF_B2 + q];
+        m.w3[q] = P[OFF_W3 + q];
+    }
+    if (threadIdx.x == 0) m.b3 = P[OFF_B3];
+    for (int q = threadIdx.x; q < MLP_W2OP; q += blockDim.x) m.w2op[q] = w2op[q];
+}
+
+__device__ __forceinline__ float mlp_wave64(const MlpLds &m, float x)
+{
+    return mlp_wave64(m.w1, m.b1, m.b2, m.w3, m.b3, m.w2op, x);
 }
 
 } // namespace fss

@@ -28,25 +28,20 @@ struct KernelArgs {
     const double *tab;
     int64_t M;
     const float *P;
+    const half4v *w2op; // the network's hidden layer in matrix-core operand order
     double A, mu, sigma, g0;
     float log_smin, inv_span;
     const double *c;
     int N;
 };
 
-// Ps: the network parameters staged in LDS (KIND 1)
-template <int KIND> __device__ inline double gfun(const KernelArgs &k, const float *Ps, double s)
+// the series kernels (KIND 0: table, 2: Clenshaw); the learned kernel has
+// its own matrix-core kernels below
+template <int KIND> __device__ inline double gfun(const KernelArgs &k, double s)
 {
+    static_assert(KIND == 0 || KIND == 2, "series kernels only");
     if (KIND == 0) return table_eval(k.tab, k.M, s);
-    if (KIND == 1) return mlp_kernel(Ps, s, k.A, k.mu, k.sigma, k.g0, k.log_smin, k.inv_span);
     return legendre_sum(k.c, k.N, 1.0 - 2.0 * s * s);
-}
-
-template <int KIND> __device__ inline void stage_params(const KernelArgs &k, float *Ps)
-{
-    if (KIND == 1)
-        for (int q = threadIdx.x; q < NPARAM; q += blockDim.x) Ps[q] = k.P[q];
-    __syncthreads();
 }
 
 __device__ inline double half_dist(const double4 &a, const double4 &b)
@@ -65,8 +60,6 @@ __global__ void __launch_bounds__(64) k_sal_near(const NearItem *__restrict__ it
 {
     __shared__ double4 lds[NCH_S];
     __shared__ double red[64];
-    __shared__ float Ps[KIND == 1 ? NPARAM : 1];
-    stage_params<KIND>(ka, Ps);
     const NearItem it = items[blockIdx.x];
     const int lane = threadIdx.x;
     const int cnt = it.tgt_cnt;
@@ -86,7 +79,7 @@ __global__ void __launch_bounds__(64) k_sal_near(const NearItem *__restrict__ it
         if (act)
             for (int k = ph; k < m; k += P) {
                 const double4 s = lds[k];
-                acc = fma(gfun<KIND>(ka, Ps, half_dist(xt, s)), s.w, acc);
+                acc = fma(gfun<KIND>(ka, half_dist(xt, s)), s.w, acc);
             }
         __syncthreads();
     }
@@ -112,8 +105,6 @@ __global__ void __launch_bounds__(64) k_sal_proxy(const ProxyItem *__restrict__ 
     constexpr int P = 64 / PP;
     __shared__ double4 lds[NCH_S];
     __shared__ double red[64];
-    __shared__ float Ps[KIND == 1 ? NPARAM : 1];
-    stage_params<KIND>(ka, Ps);
     const int64_t item = blockIdx.x;
     const ProxyItem it = items[item];
     const int lane = threadIdx.x;
@@ -133,7 +124,7 @@ __global__ void __launch_bounds__(64) k_sal_proxy(const ProxyItem *__restrict__ 
         if (act)
             for (int kk = ph; kk < m; kk += P) {
                 const double4 s = lds[kk];
-                acc = fma(gfun<KIND>(ka, Ps, half_dist(yk, s)), s.w, acc);
+                acc = fma(gfun<KIND>(ka, half_dist(yk, s)), s.w, acc);
             }
         __syncthreads();
     }
@@ -142,6 +133,132 @@ __global__ void __launch_bounds__(64) k_sal_proxy(const ProxyItem *__restrict__ 
     if (ph == 0 && act) {
         double a = 0.0;
         for (int h = 0; h < P; h++) a += red[h * PP + k];
+        phi[item * PP + k] = a;
+    }
+}
+
+// ------------------------------------------------------------------
+// learned kernel on the matrix cores
+// ------------------------------------------------------------------
+// Four items per block (one per wave) share the network parameters staged in
+// LDS; every wave stages its own source chunks.  The lanes keep the (target
+// or proxy, phase) layout of the kernels above, but the source loop runs a
+// wave-uniform number of trips so that every network evaluation (64 pairs at
+// once) sees the whole wave; lanes without a pair in a trip evaluate a dummy
+// input and add nothing.
+constexpr int NCH_M = 128;
+
+__device__ inline void sal_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// G(s) of the learned kernel from the network output y (s = 0: self term)
+__device__ inline double learned_g(const KernelArgs &k, double s, float y)
+{
+    return s > 0.0 ? k.A * (0.5 / s - 1.0) + k.mu + k.sigma * (double)y : k.g0;
+}
+
+__global__ void __launch_bounds__(256) k_sal_near_mlp(const NearItem *__restrict__ items, int64_t nitems,
+                                                      const int32_t *__restrict__ near_src,
+                                                      const double4 *__restrict__ src, KernelArgs ka, double *out)
+{
+    __shared__ MlpLds Ml;
+    __shared__ double4 stage[4][NCH_M];
+    __shared__ double red[4][64];
+    mlp_stage(Ml, ka.P, ka.w2op);
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t item = (int64_t)blockIdx.x * 4 + wave;
+    if (item >= nitems) return; // the whole wave; no block barrier follows
+    double4 *lds = stage[wave];
+    const NearItem it = items[item];
+    const int lane = threadIdx.x & 63;
+    const int cnt = it.tgt_cnt;
+    const int P = 64 / cnt;
+    const int ph = lane / cnt, tl = lane - ph * cnt;
+    const bool act = ph < P;
+    const double4 xt = src[it.tgt_start + (act ? tl : 0)];
+    double acc = 0.0;
+    const int32_t *list = near_src + it.src_off;
+    const int total = it.n_other + it.n_self;
+    for (int base = 0; base < total; base += NCH_M) {
+        const int m = min(NCH_M, total - base);
+#pragma unroll
+        for (int s = 0; s < NCH_M / 64; s++)
+            if (lane + 64 * s < m) lds[lane + 64 * s] = src[list[base + lane + 64 * s]];
+        sal_wave_sync();
+        const int trips = (m + P - 1) / P;
+        for (int t = 0; t < trips; t++) {
+            const int k = ph + t * P;
+            const bool ok = act && k < m;
+            const double4 sv = lds[ok ? k : 0];
+            const double s = half_dist(xt, sv);
+            const float y = mlp_wave64(Ml, mlp_input(s > 0.0 ? s : 1.0, ka.log_smin, ka.inv_span));
+            if (ok) acc = fma(learned_g(ka, s, y), sv.w, acc);
+        }
+        sal_wave_sync();
+    }
+    red[wave][lane] = acc;
+    sal_wave_sync();
+    if (ph == 0 && act) {
+        double a = 0.0;
+        for (int h = 0; h < P; h++) a += red[wave][h * cnt + tl];
+        out[it.out_base + tl] = a;
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_sal_proxy_mlp(const ProxyItem *__restrict__ items, int64_t nitems,
+                                                       const int64_t *__restrict__ far_off,
+                                                       const int32_t *__restrict__ far_src,
+                                                       const double4 *__restrict__ src, int64_t p0, KernelArgs ka,
+                                                       double *phi)
+{
+    constexpr int PP = npts_s<D>();
+    constexpr int P = 64 / PP;
+    __shared__ MlpLds Ml;
+    __shared__ double4 stage[4][NCH_M];
+    __shared__ double red[4][64];
+    mlp_stage(Ml, ka.P, ka.w2op);
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t item = (int64_t)blockIdx.x * 4 + wave;
+    if (item >= nitems) return;
+    double4 *lds = stage[wave];
+    const ProxyItem it = items[item];
+    const int lane = threadIdx.x & 63;
+    const int ph = lane / PP, k = lane - ph * PP;
+    const bool act = ph < P;
+    const double4 yk = src[p0 + (int64_t)it.node * PP + (act ? k : 0)];
+    double acc = 0.0;
+    const int64_t o0 = far_off[item];
+    const int total = (int)(far_off[item + 1] - o0);
+    const int32_t *list = far_src + o0;
+    for (int base = 0; base < total; base += NCH_M) {
+        const int m = min(NCH_M, total - base);
+#pragma unroll
+        for (int s = 0; s < NCH_M / 64; s++)
+            if (lane + 64 * s < m) lds[lane + 64 * s] = src[list[base + lane + 64 * s]];
+        sal_wave_sync();
+        const int trips = (m + P - 1) / P;
+        for (int t = 0; t < trips; t++) {
+            const int kk = ph + t * P;
+            const bool ok = act && kk < m;
+            const double4 sv = lds[ok ? kk : 0];
+            const double s = half_dist(yk, sv);
+            const float y = mlp_wave64(Ml, mlp_input(s > 0.0 ? s : 1.0, ka.log_smin, ka.inv_span));
+            if (ok) acc = fma(learned_g(ka, s, y), sv.w, acc);
+        }
+        sal_wave_sync();
+    }
+    red[wave][lane] = acc;
+    sal_wave_sync();
+    if (ph == 0 && act) {
+        double a = 0.0;
+        for (int h = 0; h < P; h++) a += red[wave][h * PP + k];
         phi[item * PP + k] = a;
     }
 }
@@ -301,13 +418,26 @@ template <int KIND, int D> fss_status run(fss_ctx *ctx, const KernelArgs &ka, do
     FSS_TRY(ctx->phi.ensure(sizeof(double) * p * std::max<int64_t>(1, T.n_proxy)));
     FSS_TRY(ctx->alpha.ensure(sizeof(double) * p * std::max<int64_t>(1, T.n_slot)));
     FSS_TRY(ctx->acc3.ensure(sizeof(double) * std::max<int64_t>(1, p0)));
-    if (T.n_near > 0)
-        hipLaunchKernelGGL((k_sal_near<KIND>), dim3((unsigned)T.n_near), dim3(64), 0, s, T.near_items.as<NearItem>(),
-                           T.near_src.as<int32_t>(), src, ka, ctx->out3.as<double>());
-    if (T.n_proxy > 0)
-        hipLaunchKernelGGL((k_sal_proxy<KIND, D>), dim3((unsigned)T.n_proxy), dim3(64), 0, s,
-                           T.proxy_items.as<ProxyItem>(), T.far_off.as<int64_t>(), T.far_src.as<int32_t>(), src, p0,
-                           ka, ctx->phi.as<double>());
+    if constexpr (KIND == 1) {
+        // the learned kernel: network evaluations on the matrix cores, four
+        // items per block
+        if (T.n_near > 0)
+            hipLaunchKernelGGL(k_sal_near_mlp, dim3((unsigned)div_up(T.n_near, 4)), dim3(256), 0, s,
+                               T.near_items.as<NearItem>(), T.n_near, T.near_src.as<int32_t>(), src, ka,
+                               ctx->out3.as<double>());
+        if (T.n_proxy > 0)
+            hipLaunchKernelGGL((k_sal_proxy_mlp<D>), dim3((unsigned)div_up(T.n_proxy, 4)), dim3(256), 0, s,
+                               T.proxy_items.as<ProxyItem>(), T.n_proxy, T.far_off.as<int64_t>(),
+                               T.far_src.as<int32_t>(), src, p0, ka, ctx->phi.as<double>());
+    } else {
+        if (T.n_near > 0)
+            hipLaunchKernelGGL((k_sal_near<KIND>), dim3((unsigned)T.n_near), dim3(64), 0, s,
+                               T.near_items.as<NearItem>(), T.near_src.as<int32_t>(), src, ka, ctx->out3.as<double>());
+        if (T.n_proxy > 0)
+            hipLaunchKernelGGL((k_sal_proxy<KIND, D>), dim3((unsigned)T.n_proxy), dim3(64), 0, s,
+                               T.proxy_items.as<ProxyItem>(), T.far_off.as<int64_t>(), T.far_src.as<int32_t>(), src,
+                               p0, ka, ctx->phi.as<double>());
+    }
     if (T.n_slot > 0)
         hipLaunchKernelGGL((k_sal_solve<D>), dim3((unsigned)div_up(T.n_slot, 128)), dim3(128), 0, s,
                            T.proxy_slots.as<ProxySlot>(), T.n_slot, ctx->phi.as<double>(), ctx->d_lu.as<double>(),
@@ -347,10 +477,12 @@ fss_status sal_tree(fss_ctx *ctx, const fss_particles *pp, int kind, double *out
     if (!S.c.p) return set_error("sal: series not set"), FSS_ERR_STATE;
     if (kind == 1 && !S.has_mlp) return set_error("sal: no network"), FSS_ERR_STATE;
     if (kind < 0 || kind > 2) return set_error("sal: kind 0 (table), 1 (learned) or 2 (series)"), FSS_ERR_INVALID;
+    if (kind == 1) FSS_TRY(sal_mlp_prepare(ctx));
     KernelArgs ka;
     ka.tab = S.tab.as<double>();
     ka.M = S.M;
     ka.P = S.P.as<float>();
+    ka.w2op = S.w2op.as<half4v>();
     ka.A = S.A;
     ka.mu = S.mu;
     ka.sigma = S.sigma;
@@ -366,7 +498,10 @@ fss_status sal_tree(fss_ctx *ctx, const fss_particles *pp, int kind, double *out
         FSS_HIP(hipEventRecord(ev[0], ctx->stream));
     }
     FSS_TRY(tree_sources64(ctx, pp)); // records w = eta A and the modified charges
-    if (ev) FSS_HIP(hipEventRecord(ev[1], ctx->stream));
+    if (ev) {
+        FSS_HIP(hipEventRecord(ev[4], ctx->stream));
+        FSS_HIP(hipEventRecord(ev[1], ctx->stream));
+    }
     fss_status st = kind == 0 ? run_kind<0>(ctx, ka, out) : kind == 1 ? run_kind<1>(ctx, ka, out)
                                                                     : run_kind<2>(ctx, ka, out);
     if (ev) {

@@ -276,6 +276,20 @@ __device__ inline int32_t node_count(const int32_t *counts, int lev, int idx)
     return counts[level_offset(lev) + idx];
 }
 
+// separation (r_t + r_s) / distance of the MAC (fast_rhs.h:37-41); a pair
+// within FSS_MAC_TIE of theta is decided with the correctly rounded distance,
+// so the list does not depend on the last bit of the device's acos
+__device__ inline void mac_separation(const double *ti, const double *si, double radius, double theta,
+                                      double &distance, double &separation)
+{
+    distance = fssg::great_circ_dist(ti, si, radius);
+    separation = (ti[3] + si[3]) / distance;
+    if (distance > 0 && fabs(separation - theta) <= FSS_MAC_TIE) {
+        distance = fssg::great_circ_dist_cr(ti, si, radius);
+        separation = (ti[3] + si[3]) / distance;
+    }
+}
+
 __global__ void k_trav_classify(const int4 *__restrict__ F, int64_t nF,
                                 const int32_t *__restrict__ counts,
                                 const double *__restrict__ tri_info, double theta, int many_count,
@@ -292,8 +306,8 @@ __global__ void k_trav_classify(const int4 *__restrict__ F, int64_t nF,
         const double *ti = tri_info + 4 * (level_offset(lt) + ct);
         const double *si = tri_info + 4 * (level_offset(ls) + cs);
         if (nt != 0 && ns != 0) {
-            double distance = fssg::great_circ_dist(ti, si, radius);
-            double separation = (ti[3] + si[3]) / distance;
+            double distance, separation;
+            mac_separation(ti, si, radius, theta, distance, separation);
             if (distance > 0) mb = (unsigned long long)__double_as_longlong(fabs(separation - theta));
         }
     }
@@ -310,8 +324,8 @@ __global__ void k_trav_classify(const int4 *__restrict__ F, int64_t nF,
     if (nt != 0 && ns != 0) {
         const double *ti = tri_info + 4 * (level_offset(lt) + ct);
         const double *si = tri_info + 4 * (level_offset(ls) + cs);
-        double distance = fssg::great_circ_dist(ti, si, radius);
-        double separation = (ti[3] + si[3]) / distance;
+        double distance, separation;
+        mac_separation(ti, si, radius, theta, distance, separation);
         if ((distance > 0) && (separation < theta)) {
             a = (int8_t)((nt > many_count ? 2 : 0) + (ns > many_count ? 1 : 0));
         } else if ((nt < many_count) && (ns < many_count)) {

@@ -89,6 +89,7 @@ SIGNATURES = {
     "fss_last_timing": (_int, [_vp, C.POINTER(_dbl), C.POINTER(_dbl)]),
     "fss_timing_reset": (_int, [_vp]),
     "fss_timing_mean": (_int, [_vp, C.POINTER(_i64), C.POINTER(_dbl), C.POINTER(_dbl)]),
+    "fss_timing_far_mean": (_int, [_vp, C.POINTER(_dbl)]),
     "fss_grid_icosahedral": (_int, [_int, _vp, _vp, _vp, _vp]),
     "fss_grid_voronoi_areas": (_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp]),
     "fss_grid_third_areas": (_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp]),
@@ -521,3 +522,9 @@ class Context:
         n, a, b = C.c_int64(), C.c_double(), C.c_double()
         _check(lib().fss_timing_mean(self.h, C.byref(n), C.byref(a), C.byref(b)))
         return n.value, a.value, b.value
+
+    def timing_far_mean(self):
+        """Mean far-field (proxy value) kernel ms since timing_reset()."""
+        f = C.c_double()
+        _check(lib().fss_timing_far_mean(self.h, C.byref(f)))
+        return f.value

@@ -239,6 +239,9 @@ fss_status fss_last_timing(fss_ctx *ctx, double *stage_ms, double *main_kernel_m
  * mean over the calls since (at most 256). */
 fss_status fss_timing_reset(fss_ctx *ctx);
 fss_status fss_timing_mean(fss_ctx *ctx, int64_t *calls, double *stage_ms, double *main_kernel_ms);
+/* Mean duration of the tree stage's far-field (proxy value) kernel over the
+ * same window (0 for calls without one, e.g. the direct sum). */
+fss_status fss_timing_far_mean(fss_ctx *ctx, double *far_kernel_ms);
 
 /* ---- grids and fields (host functions, host buffers) ----
  * Icosahedral particle grid after `level` refinements: V = 10*4^level + 2

@@ -51,6 +51,7 @@
  */
 #include <float.h>
 #include <math.h>
+#include <quadmath.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -179,6 +180,23 @@ static double great_circ_dist(const double *p1, const double *p2, double radius)
     if (1.0 < cl) cl = 1.0;   /* min(., 1.0)  */
     return acos(cl) * radius;
 }
+
+/* The MAC's near ties: a pair whose separation lies within MAC_TIE of theta
+ * is decided with the arccosine rounded correctly to double -- here the
+ * 113-bit acosq of libquadmath rounded to nearest -- so the decision does
+ * not depend on the last bit of a libm's acos (the reference's is the
+ * platform's). */
+#define MAC_TIE 1e-12
+static double great_circ_dist_cr(const double *p1, const double *p2, double radius)
+{
+    double s = dot3(p1, p2) / (norm3(p1) * norm3(p2));
+    double cl = s;
+    if (cl < -1.0) cl = -1.0;
+    if (1.0 < cl) cl = 1.0;
+    return (double)acosq((__float128)cl) * radius;
+}
+
+ORC_API double orc_acos_cr(double x) { return (double)acosq((__float128)x); }
 
 /* helpers.h:94-115 (L'Huilier) */
 static double sphere_tri_area(const double *p1, const double *p2, const double *p3, double radius)
@@ -560,6 +578,10 @@ ORC_API int64_t orc_tree_traverse(int tree_levels, const double *tri_info, const
         const double *ti = tri_info + 4 * it, *si = tri_info + 4 * is;
         double distance = great_circ_dist(ti, si, radius);
         double separation = (ti[3] + si[3]) / distance;
+        if (distance > 0 && fabs(separation - theta) <= MAC_TIE) {
+            distance = great_circ_dist_cr(ti, si, radius);
+            separation = (ti[3] + si[3]) / distance;
+        }
         if (distance > 0) {
             double mg = fabs(separation - theta);
             if (mg < margin) margin = mg;

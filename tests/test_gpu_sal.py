@@ -184,3 +184,84 @@ def test_sal_treecode_learned_kernel(fsslib, sal):
     e_sing = np.linalg.norm(sing - ref) / np.linalg.norm(ref)
     print("treecode rel-L2 learned", e_ml, "singular part only", e_sing)
     assert e_ml < e_sing
+
+
+def _random_network(seed):
+    rng = np.random.default_rng(seed)
+    p = (rng.normal(size=S.NPARAM) * 0.3).astype(np.float32)
+    p[:S.H] *= 4.0  # W1: the input spans [-1.5, 1], so the first layer saturates in places
+    return p
+
+
+def _network_kernel(p, s, A, mu, sigma, s_min):
+    """The learned kernel G(s) with network p, evaluated by the numpy oracle
+    (input log s mapped onto [-1, 1], clipped at -1.5, as the device maps it)."""
+    x = np.maximum(2 * (np.log(s) - np.log(s_min)) / -np.log(s_min) - 1, -1.5)
+    y, _ = S.mlp_forward(p.astype(np.float64), x)
+    return S.singular_part(s, A) + mu + sigma * y
+
+
+def test_matrix_core_network_matches_oracle(fsslib, sal):
+    """The network evaluated 64 inputs per wave on the matrix cores (fp16
+    hi/lo split of the hidden layer's operands, FP32 accumulation) against
+    the FP64 numpy forward pass with the same parameters."""
+    ctx, c, A, torch = sal
+    p = _random_network(6)
+    ctx.sal_mlp_params(p)
+    mu, sigma, s_min = 0.25, 3.0, 1e-3
+    ctx.sal_mlp_norm(mu, sigma, s_min)
+    s = np.concatenate([np.geomspace(1e-4, 1, 3001), np.random.default_rng(1).uniform(0, 1, 997)])
+    print(f"network on the matrix cores at {len(s)} separations", flush=True)
+    g = ctx.sal_kernel(torch.from_numpy(s), kind=1).cpu().numpy()
+    want = _network_kernel(p, s, A, mu, sigma, s_min)
+    ydiff = np.abs(g - want) / sigma
+    print("max |y - y_oracle|", ydiff.max())
+    # products of about 22 significant bits summed in FP32 over 64 terms, FP32
+    # tanh: network outputs of order one agree to a few 1e-6
+    assert ydiff.max() < 2e-5
+
+
+def test_learned_direct_sum_matches_oracle(fsslib, sal):
+    """SAL direct sum with the learned kernel (every source evaluated for 64
+    targets at once on the matrix cores) against the numpy oracle's direct
+    sum with the same network."""
+    ctx, c, A, torch = sal
+    p = _random_network(7)
+    ctx.sal_mlp_params(p)
+    mu, sigma, s_min = -0.1, 1.5, 1e-3
+    ctx.sal_mlp_norm(mu, sigma, s_min)
+    x, y, z, _, area, _ = icos_case(fsslib, 4)
+    eta = np.cos(2 * x) * np.sin(3 * y) + 0.3 * z
+    idx = fsslib.sample_targets(len(x), 300, 9)
+    d = [torch.from_numpy(a).cuda() for a in (x, y, z, eta, area)]
+    print(f"learned sal direct N={len(x)}", flush=True)
+    got = ctx.sal_direct(*d, idx=torch.from_numpy(idx), kind=1).cpu().numpy()
+    g0 = float(np.sum(c[1:]))  # the self term: the truncated series at theta = 0
+    want = np.zeros(len(idx))
+    for q, i in enumerate(idx):
+        s = 0.5 * np.sqrt((x - x[i]) ** 2 + (y - y[i]) ** 2 + (z - z[i]) ** 2)
+        s[i] = 1.0
+        gk = _network_kernel(p, s, A, mu, sigma, s_min)
+        gk[i] = g0
+        want[q] = np.sum(gk * eta * area)
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print("rel-L2", err)
+    assert err < 1e-6
+
+
+def test_learned_treecode_converges_to_learned_direct(fsslib, sal):
+    """The treecode's matrix-core near and proxy kernels with a learned
+    kernel: against the direct sum with the same kernel, the error falls with
+    the interpolation degree (the network part is smooth in s)."""
+    _, c, A, torch = sal
+    errs = []
+    for degree in (2, 4):
+        ctx, d, _ = _sal_tree_case(fsslib, torch, c, A, degree, 0.5)
+        ctx.sal_mlp_params(_random_network(8))
+        ctx.sal_mlp_norm(0.0, 1.0, 1e-3)
+        print(f"learned treecode degree={degree}", flush=True)
+        got = ctx.sal_tree(*d, kind=1).cpu().numpy()
+        ref = ctx.sal_direct(*d, kind=1).cpu().numpy()
+        errs.append(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print("learned treecode rel-L2 vs learned direct, degree 2/4:", errs)
+    assert errs[1] < 0.5 * errs[0] and errs[1] < 1e-3

@@ -525,7 +525,7 @@ def test_cp_pin_fails_on_mutated_oracle(orc, fsslib, tmp_path, name):
     c_file, so = tmp_path / "mutant.c", tmp_path / "libmutant.so"
     c_file.write_text(src[:a] + body.replace(old, new) + src[b:])
     subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=gnu11", "-shared",
-                    "-o", str(so), str(c_file), "-lm"], check=True)
+                    "-o", str(so), str(c_file), "-lquadmath", "-lm"], check=True)
     conv, interp = _cp_pin_errors(orc, fsslib, library=orc.load(str(so)))
     assert not _cp_pin_holds(conv, interp), (name, conv, interp)
 
