@@ -263,6 +263,7 @@ fss_status fss_create(fss_ctx **out, const fss_config *cfg)
             return set_error("no HIP device visible"), FSS_ERR_NODEVICE;
         if (cfg->device < 0 || cfg->device >= ndev) return set_error("device out of range"), FSS_ERR_INVALID;
         FSS_HIP(hipSetDevice(cfg->device));
+        FSS_TRY(direct_scratch_setup(cfg->device));
         fss_ctx *c = new fss_ctx();
         c->cfg = *cfg;
         if (hipStreamCreate(&c->stream) != hipSuccess) {
@@ -422,6 +423,14 @@ fss_status fss_velocity_tree(fss_ctx *ctx, const fss_particles *p, fss_velocity 
     FSS_GUARD({
         if (!ctx || bad_particles(p) || bad_velocity(out)) return set_error("null argument"), FSS_ERR_INVALID;
         return velocity_tree(ctx, p, out);
+    })
+}
+
+fss_status fss_velocity_direct_noalloc(fss_ctx *ctx, const fss_particles *p, fss_velocity *out)
+{
+    FSS_GUARD({
+        if (!ctx || bad_particles(p) || bad_velocity(out)) return set_error("null argument"), FSS_ERR_INVALID;
+        return velocity_direct_noalloc(ctx, p, out);
     })
 }
 

@@ -16,6 +16,7 @@
 //       accumulators (256 terms per partial), the cross product and scaling
 //       are FP64.
 #include <cmath>
+#include <mutex>
 
 #include "fss_internal.h"
 
@@ -203,6 +204,80 @@ fss_status velocity_direct(fss_ctx *ctx, const fss_particles *p, const int64_t *
     if (!ctx->capturing) FSS_HIP(hipEventRecord(ev[2], s));
     if (!ctx->capturing) FSS_HIP(hipEventRecord(ev[3], s));
     if (!ctx->capturing) ctx->timing.valid = true;
+    FSS_HIP(hipGetLastError());
+    return FSS_OK;
+}
+
+// ------------------------------------------------------------------
+// allocation-free variant: the packed source records live in a file-scope
+// device array instead of the context's workspace
+// ------------------------------------------------------------------
+// One copy per GPU, held until the process ends.  Every call on a GPU packs
+// its records into that GPU's copy, so calls that overlap on one GPU (two
+// streams, two threads) race; calls in one stream order are safe.  Nothing in
+// it needs clearing: a call writes records [0, n) before reading them.
+static __device__ __align__(256) unsigned char scratch[FSS_DIRECT_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch[SCRATCH_MAX_DEVICES] = {};
+
+fss_status direct_scratch_setup(int device)
+{
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) {
+        set_error("direct scratch: device index beyond " + std::to_string(SCRATCH_MAX_DEVICES));
+        return FSS_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return FSS_OK;
+    int cur = -1;
+    FSS_HIP(hipGetDevice(&cur));
+    if (cur != device) {
+        set_error("direct scratch: set-up needs its device current");
+        return FSS_ERR_STATE;
+    }
+    void *ptr = nullptr;
+    FSS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) {
+        set_error("direct scratch: no device address");
+        return FSS_ERR_HIP;
+    }
+    g_scratch[device] = ptr;
+    return FSS_OK;
+}
+
+fss_status velocity_direct_noalloc(fss_ctx *ctx, const fss_particles *p, fss_velocity *out)
+{
+    hipStream_t s = ctx->stream;
+    const int64_t n = p->n;
+    const bool mixed = ctx->cfg.precision == FSS_MIXED;
+    const size_t rec = mixed ? sizeof(float4) : sizeof(double4);
+    if (n < 1 || (size_t)n > FSS_DIRECT_SCRATCH_BYTES / rec) {
+        set_error("velocity_direct_noalloc: n must be in [1, " +
+                  std::to_string(FSS_DIRECT_SCRATCH_BYTES / rec) + "] in " + (mixed ? "mixed" : "FP64") +
+                  " precision (" + std::to_string(FSS_DIRECT_SCRATCH_BYTES >> 20) + " MiB of scratch), got " +
+                  std::to_string(n));
+        return FSS_ERR_INVALID;
+    }
+    const int dev = ctx->cfg.device;
+    void *buf = (dev >= 0 && dev < SCRATCH_MAX_DEVICES) ? g_scratch[dev] : nullptr;
+    if (!buf) {
+        set_error("velocity_direct_noalloc: scratch of device " + std::to_string(dev) + " not set up");
+        return FSS_ERR_STATE;
+    }
+    const double scale = -1.0 / (4.0 * M_PI);
+    unsigned gp = (unsigned)div_up(n, 256), gt = (unsigned)div_up(n, DTILE);
+    if (!mixed) {
+        double4 *src = static_cast<double4 *>(buf);
+        hipLaunchKernelGGL(k_pack64, dim3(gp), dim3(256), 0, s, p->x, p->y, p->z, p->vort, p->area, n, src);
+        hipLaunchKernelGGL(k_direct64, dim3(gt), dim3(DTILE), 0, s, src, n, nullptr, n, scale, ctx->cfg.omega,
+                           out->u, out->v, out->w, out->dvort);
+    } else {
+        float4 *src = static_cast<float4 *>(buf);
+        hipLaunchKernelGGL(k_pack32, dim3(gp), dim3(256), 0, s, p->x, p->y, p->z, p->vort, p->area, n, src);
+        hipLaunchKernelGGL(k_direct32, dim3(gt), dim3(DTILE), 0, s, src, n, nullptr, n, scale, ctx->cfg.omega,
+                           out->u, out->v, out->w, out->dvort);
+    }
     FSS_HIP(hipGetLastError());
     return FSS_OK;
 }

@@ -309,6 +309,12 @@ fss_status velocity_tree(fss_ctx *ctx, const fss_particles *p, fss_velocity *out
 fss_status velocity_direct(fss_ctx *ctx, const fss_particles *p, const int64_t *idx, int64_t m,
                            fss_velocity *out, bool force_fp64);
 
+// allocation-free direct sum (fss_direct.hip): records packed into a
+// file-scope device array of FSS_DIRECT_SCRATCH_BYTES per GPU, whose address
+// direct_scratch_setup resolves once per device (that device current)
+fss_status direct_scratch_setup(int device);
+fss_status velocity_direct_noalloc(fss_ctx *ctx, const fss_particles *p, fss_velocity *out);
+
 // ---- RK4 (fss_rk4.hip) ----
 fss_status rk4_step(fss_ctx *ctx, double *x, double *y, double *z, double *vort,
                     const double *area, int64_t n, double dt, int use_tree);

@@ -83,6 +83,7 @@ SIGNATURES = {
     "fss_velocity_tree": (_int, [_vp, C.POINTER(Particles), C.POINTER(Velocity)]),
     "fss_velocity_direct_targets": (_int, [_vp, C.POINTER(Particles), _vp, _i64,
                                            C.POINTER(Velocity)]),
+    "fss_velocity_direct_noalloc": (_int, [_vp, C.POINTER(Particles), C.POINTER(Velocity)]),
     "fss_rk4_step": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, _int]),
     "fss_rk4_run_graph": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, _int, _int]),
     "fss_project_points": (_int, [_vp, _vp, _vp, _vp, _vp, _i64]),
@@ -392,6 +393,14 @@ class Context:
         o, ov = self._out(x.numel(), out)
         p = self._particles(x, y, z, vort, area)
         _check(lib().fss_velocity_direct(self.h, C.byref(p), C.byref(ov)))
+        return tuple(o)
+
+    def velocity_direct_noalloc(self, x, y, z, vort, area, out=None):
+        """velocity_direct with the records in the library's per-GPU scratch
+        (no allocation in the call; n at most 2^20 in FP64, 2^21 mixed)."""
+        o, ov = self._out(x.numel(), out)
+        p = self._particles(x, y, z, vort, area)
+        _check(lib().fss_velocity_direct_noalloc(self.h, C.byref(p), C.byref(ov)))
         return tuple(o)
 
     def velocity_tree(self, x, y, z, vort, area, out=None):

@@ -185,6 +185,19 @@ fss_status fss_velocity_tree(fss_ctx *ctx, const fss_particles *p, fss_velocity 
  * accuracy reference on sampled targets. */
 fss_status fss_velocity_direct_targets(fss_ctx *ctx, const fss_particles *p, const int64_t *idx,
                                        int64_t m, fss_velocity *out);
+/* fss_velocity_direct without any allocation in the call.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The packed source records go to a per-GPU scratch array of
+ * FSS_DIRECT_SCRATCH_BYTES that the library holds until the process ends
+ * (its address is resolved in fss_create), so n is at most
+ * FSS_DIRECT_SCRATCH_BYTES / 32 in FP64 and / 16 in mixed precision
+ * (1,048,576 and 2,097,152); larger n fails with FSS_ERR_INVALID.  Every call
+ * on one GPU shares that scratch: calls that overlap on one GPU (other
+ * streams, other threads) race.  Results equal fss_velocity_direct's bit for
+ * bit; no timing events are recorded. */
+#define FSS_DIRECT_SCRATCH_BYTES ((uint64_t)32 << 20)
+fss_status fss_velocity_direct_noalloc(fss_ctx *ctx, const fss_particles *p, fss_velocity *out);
 
 /* ---- RK4 step with projection (new_dynamics.cpp:142-238 intent;
  * project_points helpers.h:362-372) on device state x, y, z, vort (updated in
