@@ -163,11 +163,17 @@ fss_status interp_setup(fss_ctx *ctx)
     hipStream_t s = ctx->stream;
     FSS_TRY(ctx->d_lu.ensure(sizeof(double) * p * p));
     FSS_TRY(ctx->d_vinv.ensure(sizeof(double) * p * p));
-    FSS_TRY(ctx->d_ipiv.ensure(sizeof(int32_t) * p));
+    // the row interchanges of dgetrs (ipiv applied in sequence) as one
+    // permutation: b_permuted[k] = b[perm[k]]
+    std::vector<int32_t> perm(p);
+    for (int k = 0; k < p; k++) perm[k] = k;
+    for (int k = 0; k < p; k++)
+        if (I.ipiv[k] != k) std::swap(perm[k], perm[I.ipiv[k]]);
+    FSS_TRY(ctx->d_perm.ensure(sizeof(int32_t) * p));
     FSS_TRY(ctx->d_pts.ensure(sizeof(double) * 3 * p));
     FSS_HIP(hipMemcpyAsync(ctx->d_lu.p, I.lu.data(), sizeof(double) * p * p, hipMemcpyHostToDevice, s));
     FSS_HIP(hipMemcpyAsync(ctx->d_vinv.p, I.vinv.data(), sizeof(double) * p * p, hipMemcpyHostToDevice, s));
-    FSS_HIP(hipMemcpyAsync(ctx->d_ipiv.p, I.ipiv.data(), sizeof(int32_t) * p, hipMemcpyHostToDevice, s));
+    FSS_HIP(hipMemcpyAsync(ctx->d_perm.p, perm.data(), sizeof(int32_t) * p, hipMemcpyHostToDevice, s));
     FSS_HIP(hipMemcpyAsync(ctx->d_pts.p, I.pts.data(), sizeof(double) * 3 * p, hipMemcpyHostToDevice, s));
     FSS_HIP(hipStreamSynchronize(s));
     return FSS_OK;
@@ -355,6 +361,21 @@ fss_status fss_autotune_levels(fss_ctx *ctx, const fss_particles *p, int lmin, i
             ctx->graph_exec = nullptr;
         }
         return autotune_levels(ctx, p, lmin, lmax, reps, best_levels, ms);
+    })
+}
+
+fss_status fss_autotune(fss_ctx *ctx, const fss_particles *p, const fss_tune_point *cand, int ncand,
+                        const int64_t *idx, int64_t m, double max_rel_l2, int reps, int *best, double *ms,
+                        double *err)
+{
+    FSS_GUARD({
+        if (!ctx || bad_particles(p) || !cand || !idx) return set_error("null argument"), FSS_ERR_INVALID;
+        FSS_HIP(hipSetDevice(ctx->cfg.device));
+        if (ctx->graph_exec) {
+            (void)hipGraphExecDestroy(ctx->graph_exec);
+            ctx->graph_exec = nullptr;
+        }
+        return autotune(ctx, p, cand, ncand, idx, m, max_rel_l2, reps, best, ms, err);
     })
 }
 

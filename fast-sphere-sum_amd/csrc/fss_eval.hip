@@ -725,6 +725,28 @@ __device__ inline void wave_sync()
 // lanes split into proxy groups x source phases, reduction over the phases
 // through LDS.
 constexpr int NCH_PROXY = 128;
+
+// Proxies per lane for p proxies: the R in 2..4 whose layout keeps the most
+// lanes busy -- lanes in whole phase groups (T = ceil(p / R) lanes per group,
+// 64 / T groups) times the share of the group's T R slots that hold a proxy --
+// the smaller R on ties.  Degree 7 (36 proxies): R = 4 keeps 63 of 64 lanes
+// busy where R = 2 keeps 54.
+constexpr int proxy_lane_r(int pp)
+{
+    int best = 2;
+    double be = -1.0;
+    for (int r = 2; r <= 4; r++) {
+        const int t = (pp + r - 1) / r, groups = 64 / t;
+        const double e = (double)(groups * t) / 64.0 * (double)pp / (double)(t * r);
+        if (e > be + 1e-12) {
+            be = e;
+            best = r;
+        }
+    }
+    return best;
+}
+static_assert(proxy_lane_r(6) == 3 && proxy_lane_r(10) == 2 && proxy_lane_r(21) == 3 && proxy_lane_r(36) == 4,
+              "proxy lane layout");
 template <typename Real, int D>
 __global__ void __launch_bounds__(256) k_proxy_list(const ProxyItem *__restrict__ items, int64_t nitems,
                                                    const int64_t *__restrict__ far_off,
@@ -737,8 +759,9 @@ __global__ void __launch_bounds__(256) k_proxy_list(const ProxyItem *__restrict_
 {
     using T4 = typename Rec<Real>::T4;
     constexpr int PP = npts<D>();
-    // proxies per lane: 3 up to degree 2, 2 beyond (fewer registers)
-    constexpr int R = PP <= 6 ? 3 : 2;
+    // proxies per lane: FP64 3 up to degree 2, 2 beyond (registers); FP32 the
+    // layout that keeps the most lanes busy
+    constexpr int R = sizeof(Real) == 4 ? proxy_lane_r(PP) : (PP <= 6 ? 3 : 2);
     constexpr int TT = (PP + R - 1) / R;  // lanes per phase group
     constexpr int NOUT = TT * 3 * R;       // phase sums: (group lane, proxy, component)
     constexpr int RED_BYTES = sizeof(double) * (64 * 3 * R + NOUT);
@@ -842,11 +865,12 @@ __global__ void __launch_bounds__(256) k_proxy_list(const ProxyItem *__restrict_
 }
 
 // sum the chunks of each node in order and solve alpha = V^{-1} phi
-// (dgetrs, fast_rhs.h:217): one thread per (slot, component)
+// (dgetrs, fast_rhs.h:217): one thread per (slot, component); the row
+// interchanges are applied while summing (perm: the composed ipiv sequence)
 template <int D>
-__global__ void k_solve(const ProxySlot *__restrict__ slots, int64_t nslot,
+__global__ void __launch_bounds__(128) k_solve(const ProxySlot *__restrict__ slots, int64_t nslot,
                         const double *__restrict__ phi, const double *__restrict__ lu,
-                        const int32_t *__restrict__ ipiv, double *alpha)
+                        const int32_t *__restrict__ perm, double *alpha)
 {
     constexpr int P = npts<D>();
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -860,22 +884,7 @@ __global__ void k_solve(const ProxySlot *__restrict__ slots, int64_t nslot,
     for (int part = sl.part_beg; part < sl.part_end; part++) {
         const double *o = phi + (int64_t)part * 3 * P + c * P;
 #pragma unroll
-        for (int k = 0; k < P; k++) b[k] += o[k];
-    }
-#pragma unroll
-    for (int k = 0; k < P; k++) {
-        int r = ipiv[k];
-        if (r != k) {
-            // runtime pivot index: swap through a short select loop
-            double bk = b[k], br = 0.0;
-#pragma unroll
-            for (int t = 0; t < P; t++)
-                if (t == r) br = b[t];
-#pragma unroll
-            for (int t = 0; t < P; t++)
-                if (t == r) b[t] = bk;
-            b[k] = br;
-        }
+        for (int k = 0; k < P; k++) b[k] += o[perm[k]];
     }
 #pragma unroll
     for (int k = 0; k < P; k++)
@@ -933,7 +942,7 @@ __device__ inline void add_interp(const double *__restrict__ tinv, const double 
 // The sums, with the rhs_func scaling (rhs_eval.h:54-55), go straight to the
 // particle's place in the caller's arrays.
 template <typename Real, int D>
-__global__ void k_combine(const int32_t *__restrict__ recid, const int32_t *__restrict__ recnode,
+__global__ void __launch_bounds__(256) k_combine(const int32_t *__restrict__ recid, const int32_t *__restrict__ recnode,
                           const uint8_t *__restrict__ rinfo, const int4 *__restrict__ nrange,
                           const int32_t *__restrict__ oslot, const int32_t *__restrict__ node_slot,
                           const int32_t *__restrict__ nodeof, const int32_t *__restrict__ inv_active,
@@ -1140,7 +1149,7 @@ static fss_status run_degree(fss_ctx *ctx, const fss_particles *pp, fss_velocity
     if (T.n_slot > 0)
         hipLaunchKernelGGL((k_solve<D>), dim3((unsigned)div_up(3 * T.n_slot, 128)), dim3(128), 0, s,
                            T.proxy_slots.as<ProxySlot>(), T.n_slot, ctx->phi.as<double>(),
-                           ctx->d_lu.as<double>(), ctx->d_ipiv.as<int32_t>(), ctx->alpha.as<double>());
+                           ctx->d_lu.as<double>(), ctx->d_perm.as<int32_t>(), ctx->alpha.as<double>());
     if (mixed)
         hipLaunchKernelGGL((k_combine<float, D>), dim3((unsigned)div_up(p0, 256)), dim3(256), 0, s,
                            T.recid.as<int32_t>(), T.recnode.as<int32_t>(), T.rinfo.as<uint8_t>(), T.nrange.as<int4>(),

@@ -225,7 +225,7 @@ struct Sal {
     double mu = 0, sigma = 1; // normalisation of the learned remainder
     double s_min = 1e-3;      // the network input maps log s from [s_min, 1] onto [-1, 1]
     DevBuf c, tab, P, Pm, Pv, w;
-    DevBuf w2op; // the hidden layer's weights in matrix-core operand order (fp16 hi/lo)
+    DevBuf mlpop; // the network in the form of its matrix-core evaluation (MlpOp, fss_sal_dev.h)
     bool has_mlp = false;
     int64_t adam_t = 0;
 };
@@ -259,7 +259,7 @@ struct fss_ctx {
     fss::Sal sal;
 
     // device constants
-    fss::DevBuf d_lu, d_ipiv, d_vinv, d_pts;
+    fss::DevBuf d_lu, d_perm, d_vinv, d_pts;
 
     // per-stage workspace
     fss::DevBuf src64;   // double4 [p0 + nn*p] or [n] for the direct sum
@@ -337,6 +337,9 @@ fss_status remesh(fss_ctx *ctx, const double *ax, const double *ay, const double
 int levels_for(int64_t n, double occupancy);
 fss_status autotune_levels(fss_ctx *ctx, const fss_particles *pp, int lmin, int lmax, int reps, int *best,
                            double *ms);
+fss_status autotune(fss_ctx *ctx, const fss_particles *pp, const fss_tune_point *cand, int ncand,
+                    const int64_t *idx, int64_t m, double max_rel_l2, int reps, int *best, double *ms,
+                    double *err);
 
 // ---- batched instances (fss_batch.cpp) ----
 fss_status batch_build(fss_ctx *ctx, int ninst, const int64_t *ns, const int *levels, const double *x,

@@ -97,23 +97,38 @@ __device__ inline double mlp_kernel(const float *__restrict__ P, double s, doubl
 // ------------------------------------------------------------------
 // the network on the matrix cores: 64 inputs per wave
 // ------------------------------------------------------------------
-// The 64 x 64 hidden layer of 64 inputs is a [64 x 64] x [64 x 64] product,
-// evaluated transposed, Z2^T = W2 H1^T, by v_mfma_f32_32x32x8_f16 on 2 x 2
-// blocks of 32 x 32 and 8 K-slices of 8.  FP32 operands are split into fp16
-// hi + lo (hi = fp16(v), lo = fp16(v - hi)) and multiplied as hi.hi + hi.lo +
-// lo.hi with FP32 accumulation: about 22 significant bits per product.
-// Operand layouts of the 32x32x8 product (one wave): A[m][k] is held by lane
-// m + 32 (k / 4) as element k % 4; B[k][n] by lane n + 32 (k / 4), element
-// k % 4; the 16 results of lane l are D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32],
-// q = 0..15.  So the result rows of one block are the K rows of the next
-// product's B operand, and H1^T can be built lane by lane: lane l computes
-// the hidden units k = 8 s + 4 (l / 32) + 0..3 of inputs l % 32 and 32 + l % 32.
-typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+// With t = tanh(z) = 1 - 2 sigma, sigma = 1 / (1 + 2^(kappa z)), kappa =
+// 2 log2(e), the network is rewritten on the sigmas of its two layers:
+//   sigma1_k = 1 / (1 + 2^(w1'_k x + b1'_k)),       w1' = kappa w1, b1' = kappa b1
+//   sigma2_j = 1 / (1 + 2^(b2'_j + sum_k W2'_jk sigma1_k)),
+//                  W2' = -2 kappa W2, b2'_j = kappa (b2_j + sum_k W2_jk)
+//   y = b3' + sum_j w3'_j sigma2_j,                 w3' = -2 w3, b3' = b3 + sum_j w3_j
+// so that an activation costs one v_exp_f32, one add and one v_rcp_f32.  The
+// 64 x 64 hidden layer of 64 inputs is a [64 x 64] x [64 x 64] product,
+// evaluated transposed, Z2^T = W2' S1^T, by v_mfma_f32_32x32x16_f16 on 2 x 2
+// blocks of 32 x 32 and 4 K-slices of 16, with the accumulators starting at
+// b2'.  FP32 operands are split into fp16 hi + lo (hi = fp16(v), lo = fp16(v -
+// hi)) and multiplied as hi.hi + hi.lo + lo.hi with FP32 accumulation: about
+// 22 significant bits per product.  Operand layouts of the 32x32x16 product
+// (one wave): A[m][k] is held by lane m + 32 (k / 8) as element k % 8, B[k][n]
+// by lane n + 32 (k / 8), element k % 8; the 16 results of lane l are
+// D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32], q = 0..15.  S1^T is built lane by
+// lane: lane l computes the units k = 16 s + 8 (l / 32) + 0..7 of inputs l % 32
+// and 32 + l % 32.
+typedef _Float16 half8v __attribute__((ext_vector_type(8)));
 typedef float float16v __attribute__((ext_vector_type(16)));
 
-// W2 in A-operand order: entry ((s * 2 + r) * 2 + part) * 64 + lane holds
-// W2[32 r + lane % 32][8 s + 4 (lane / 32) + 0..3], part 0 = hi, 1 = lo
-constexpr int MLP_W2OP = 8 * 2 * 2 * 64; // half4 entries
+constexpr float MLP_KAPPA = 2.8853900817779268f; // 2 log2(e)
+constexpr int MLP_W2OP = 4 * 2 * 2 * 64;          // half8 entries of W2' in operand order
+
+// The network's parameters in the form mlp_wave64 reads (built on the device
+// from P by sal_mlp_prepare): w2 entry ((s * 2 + r) * 2 + part) * 64 + lane
+// holds W2'[32 r + lane % 32][16 s + 8 (lane / 32) + 0..7], part 0 = hi, 1 = lo.
+struct MlpOp {
+    float w1[H], b1[H], b2[H], w3[H];
+    float b3, pad[3];
+    half8v w2[MLP_W2OP];
+};
 
 __device__ __forceinline__ void split_f16(float v, _Float16 &hi, _Float16 &lo)
 {
@@ -121,51 +136,59 @@ __device__ __forceinline__ void split_f16(float v, _Float16 &hi, _Float16 &lo)
     lo = (_Float16)(v - (float)hi);
 }
 
-// Network output for the input x of every lane of the wave (all 64 lanes
-// must be active: the matrix instructions act on the whole wave).  w1, b1, b2,
-// w3 point to the 64-entry parameter vectors, w2op to the operand table above.
-__device__ __forceinline__ float mlp_wave64(const float *__restrict__ w1, const float *__restrict__ b1,
-                                   const float *__restrict__ b2, const float *__restrict__ w3, float b3,
-                                   const half4v *__restrict__ w2op, float x)
+// 1 / (1 + 2^t): the sigma of an activation
+__device__ __forceinline__ float mlp_sigma(float t)
 {
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(t));
+}
+
+// Network output for the input x of every lane of the wave (all 64 lanes
+// must be active: the matrix instructions act on the whole wave).
+__device__ __forceinline__ float mlp_wave64(const MlpOp &m, float x)
+{
+    // the parameters are read from m (LDS) at every call: hoisted out of a
+    // caller's loop they would hold about 200 VGPRs and leave one wave per SIMD
+    __asm__ volatile("" ::: "memory");
     const int l = threadIdx.x & 63, l32 = l & 31, hh = l >> 5;
     const float xc[2] = {__shfl(x, l32, 64), __shfl(x, 32 + l32, 64)};
     float16v acc[2][2];
 #pragma unroll
     for (int r = 0; r < 2; r++)
 #pragma unroll
-        for (int c = 0; c < 2; c++)
+        for (int q = 0; q < 16; q++) {
+            const float b = m.b2[32 * r + 8 * (q >> 2) + 4 * hh + (q & 3)];
+            acc[r][0][q] = b;
+            acc[r][1][q] = b;
+        }
 #pragma unroll
-            for (int q = 0; q < 16; q++) acc[r][c][q] = 0.f;
+    for (int s = 0; s < 4; s++) {
+        half8v bh[2], bl[2];
 #pragma unroll
-    for (int s = 0; s < 8; s++) {
-        half4v bh[2], bl[2];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int k = 8 * s + 4 * hh + i;
-            const float wk = w1[k], bk = b1[k];
+        for (int i = 0; i < 8; i++) {
+            const int k = 16 * s + 8 * hh + i;
+            const float wk = m.w1[k], bk = m.b1[k];
 #pragma unroll
             for (int c = 0; c < 2; c++) {
                 _Float16 a, b;
-                split_f16(tanh_fast(fmaf(wk, xc[c], bk)), a, b);
+                split_f16(mlp_sigma(fmaf(wk, xc[c], bk)), a, b);
                 bh[c][i] = a;
                 bl[c][i] = b;
             }
         }
 #pragma unroll
         for (int r = 0; r < 2; r++) {
-            const half4v ah = w2op[((s * 2 + r) * 2 + 0) * 64 + l];
-            const half4v al = w2op[((s * 2 + r) * 2 + 1) * 64 + l];
+            const half8v ah = m.w2[((s * 2 + r) * 2 + 0) * 64 + l];
+            const half8v al = m.w2[((s * 2 + r) * 2 + 1) * 64 + l];
 #pragma unroll
             for (int c = 0; c < 2; c++) {
-                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x8f16(ah, bh[c], acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x8f16(ah, bl[c], acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x8f16(al, bh[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[c], acc[r][c], 0, 0, 0);
             }
         }
     }
-    // output layer: lane l holds hidden units j = 32 r + 8 (q / 4) + 4 hh + q % 4
-    // of inputs 32 c + l32; the other half of the units is in lane l ^ 32
+    // output layer: lane l holds units j = 32 r + 8 (q / 4) + 4 hh + q % 4 of
+    // inputs 32 c + l32; the other half of the units is in lane l ^ 32
     float part[2] = {0.f, 0.f};
 #pragma unroll
     for (int c = 0; c < 2; c++)
@@ -174,35 +197,20 @@ __device__ __forceinline__ float mlp_wave64(const float *__restrict__ w1, const 
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const int j = 32 * r + 8 * (q >> 2) + 4 * hh + (q & 3);
-                part[c] = fmaf(w3[j], tanh_fast(acc[r][c][q] + b2[j]), part[c]);
+                part[c] = fmaf(m.w3[j], mlp_sigma(acc[r][c][q]), part[c]);
             }
 #pragma unroll
     for (int c = 0; c < 2; c++) part[c] += __shfl_xor(part[c], 32, 64);
-    return (hh ? part[1] : part[0]) + b3; // lane l's input is column l32 of block hh
+    return (hh ? part[1] : part[0]) + m.b3; // lane l's input is column l32 of block hh
 }
 
-// Network parameters staged in LDS for mlp_wave64 by the threads of a block:
-// w1, b1, b2, w3 (64 floats each), b3, and the W2 operand table.
-struct MlpLds {
-    float w1[H], b1[H], b2[H], w3[H], b3;
-    half4v w2op[MLP_W2OP];
-};
-
-__device__ inline void mlp_stage(MlpLds &m, const float *__restrict__ P, const half4v *__restrict__ w2op)
+// the prepared parameters copied to LDS by the threads of a block
+__device__ inline void mlp_stage(MlpOp &lds, const MlpOp *__restrict__ op)
 {
-    for (int q = threadIdx.x; q < H; q += blockDim.x) {
-        m.w1[q] = P[OFF_W1 + q];
-        m.b1[q] = P[OFF_B1 + q];
-        m.b2[q] = P[OFF_B2 + q];
-        m.w3[q] = P[OFF_W3 + q];
-    }
-    if (threadIdx.x == 0) m.b3 = P[OFF_B3];
-    for (int q = threadIdx.x; q < MLP_W2OP; q += blockDim.x) m.w2op[q] = w2op[q];
-}
-
-__device__ __forceinline__ float mlp_wave64(const MlpLds &m, float x)
-{
-    return mlp_wave64(m.w1, m.b1, m.b2, m.w3, m.b3, m.w2op, x);
+    static_assert(sizeof(MlpOp) % 16 == 0, "MlpOp is copied in 16-byte words");
+    const int4 *src = reinterpret_cast<const int4 *>(op);
+    int4 *dst = reinterpret_cast<int4 *>(&lds);
+    for (int q = threadIdx.x; q < (int)(sizeof(MlpOp) / 16); q += blockDim.x) dst[q] = src[q];
 }
 
 } // namespace fss

@@ -28,7 +28,7 @@ struct KernelArgs {
     const double *tab;
     int64_t M;
     const float *P;
-    const half4v *w2op; // the network's hidden layer in matrix-core operand order
+    const MlpOp *mlp;   // the network's parameters in the form of the matrix-core evaluation
     double A, mu, sigma, g0;
     float log_smin, inv_span;
     const double *c;
@@ -165,10 +165,10 @@ __global__ void __launch_bounds__(256) k_sal_near_mlp(const NearItem *__restrict
                                                       const int32_t *__restrict__ near_src,
                                                       const double4 *__restrict__ src, KernelArgs ka, double *out)
 {
-    __shared__ MlpLds Ml;
+    __shared__ MlpOp Ml;
     __shared__ double4 stage[4][NCH_M];
     __shared__ double red[4][64];
-    mlp_stage(Ml, ka.P, ka.w2op);
+    mlp_stage(Ml, ka.mlp);
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t item = (int64_t)blockIdx.x * 4 + wave;
@@ -219,10 +219,10 @@ __global__ void __launch_bounds__(256) k_sal_proxy_mlp(const ProxyItem *__restri
 {
     constexpr int PP = npts_s<D>();
     constexpr int P = 64 / PP;
-    __shared__ MlpLds Ml;
+    __shared__ MlpOp Ml;
     __shared__ double4 stage[4][NCH_M];
     __shared__ double red[4][64];
-    mlp_stage(Ml, ka.P, ka.w2op);
+    mlp_stage(Ml, ka.mlp);
     __syncthreads();
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t item = (int64_t)blockIdx.x * 4 + wave;
@@ -264,10 +264,10 @@ __global__ void __launch_bounds__(256) k_sal_proxy_mlp(const ProxyItem *__restri
 }
 
 // sum the chunks of every interpolant slot in order and solve V alpha = phi
-// (one right-hand side)
+// (one right-hand side; the row interchanges applied while summing)
 template <int D>
-__global__ void k_sal_solve(const ProxySlot *__restrict__ slots, int64_t nslot, const double *__restrict__ phi,
-                            const double *__restrict__ lu, const int32_t *__restrict__ ipiv, double *alpha)
+__global__ void __launch_bounds__(128) k_sal_solve(const ProxySlot *__restrict__ slots, int64_t nslot, const double *__restrict__ phi,
+                            const double *__restrict__ lu, const int32_t *__restrict__ perm, double *alpha)
 {
     constexpr int P = npts_s<D>();
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -278,21 +278,7 @@ __global__ void k_sal_solve(const ProxySlot *__restrict__ slots, int64_t nslot, 
     for (int k = 0; k < P; k++) b[k] = 0.0;
     for (int part = sl.part_beg; part < sl.part_end; part++)
 #pragma unroll
-        for (int k = 0; k < P; k++) b[k] += phi[(int64_t)part * P + k];
-#pragma unroll
-    for (int k = 0; k < P; k++) {
-        int r = ipiv[k];
-        if (r != k) {
-            double bk = b[k], br = 0.0;
-#pragma unroll
-            for (int t = 0; t < P; t++)
-                if (t == r) br = b[t];
-#pragma unroll
-            for (int t = 0; t < P; t++)
-                if (t == r) b[t] = bk;
-            b[k] = br;
-        }
-    }
+        for (int k = 0; k < P; k++) b[k] += phi[(int64_t)part * P + perm[k]];
 #pragma unroll
     for (int k = 0; k < P; k++)
 #pragma unroll
@@ -332,7 +318,7 @@ __device__ inline double sal_interp(const double *__restrict__ tinv, const doubl
 // per main record (tree order): near slots of its main path + interpolants of
 // the nodes on it; memberships outside the main path from the per-level maps
 template <int D>
-__global__ void k_sal_combine(const int32_t *__restrict__ recid, const int32_t *__restrict__ recnode,
+__global__ void __launch_bounds__(256) k_sal_combine(const int32_t *__restrict__ recid, const int32_t *__restrict__ recnode,
                               const uint8_t *__restrict__ rinfo, const int4 *__restrict__ nrange,
                               const int32_t *__restrict__ oslot, const int32_t *__restrict__ node_slot,
                               const int32_t *__restrict__ nodeof, const int32_t *__restrict__ inv_active,
@@ -441,7 +427,7 @@ template <int KIND, int D> fss_status run(fss_ctx *ctx, const KernelArgs &ka, do
     if (T.n_slot > 0)
         hipLaunchKernelGGL((k_sal_solve<D>), dim3((unsigned)div_up(T.n_slot, 128)), dim3(128), 0, s,
                            T.proxy_slots.as<ProxySlot>(), T.n_slot, ctx->phi.as<double>(), ctx->d_lu.as<double>(),
-                           ctx->d_ipiv.as<int32_t>(), ctx->alpha.as<double>());
+                           ctx->d_perm.as<int32_t>(), ctx->alpha.as<double>());
     hipLaunchKernelGGL((k_sal_combine<D>), dim3((unsigned)div_up(p0, 256)), dim3(256), 0, s, T.recid.as<int32_t>(),
                        T.recnode.as<int32_t>(), T.rinfo.as<uint8_t>(), T.nrange.as<int4>(), T.oslot.as<int32_t>(),
                        T.node_slot.as<int32_t>(), T.nodeof.as<int32_t>(), T.inv_active.as<int32_t>(),
@@ -482,7 +468,7 @@ fss_status sal_tree(fss_ctx *ctx, const fss_particles *pp, int kind, double *out
     ka.tab = S.tab.as<double>();
     ka.M = S.M;
     ka.P = S.P.as<float>();
-    ka.w2op = S.w2op.as<half4v>();
+    ka.mlp = S.mlpop.as<MlpOp>();
     ka.A = S.A;
     ka.mu = S.mu;
     ka.sigma = S.sigma;

@@ -1,4 +1,5 @@
-// fss_tune.cpp -- leaf-size autotuner.
+// fss_tune.cpp -- autotuners: leaf size, and the treecode's parameters under
+// an accuracy bar.
 //
 // The reference takes the tree depth by hand (-l, run_config.h).  Here the
 // depth that minimises the measured stage time is picked on the device: for
@@ -12,6 +13,8 @@
 
 #include <cmath>
 #include <limits>
+#include <string>
+#include <vector>
 
 #include "fss_internal.h"
 
@@ -79,6 +82,115 @@ fss_status autotune_levels(fss_ctx *ctx, const fss_particles *pp, int lmin, int 
     ctx->cfg.tree_levels = best_L;
     if (best_L != lmax) FSS_TRY(tree_build(ctx, pp->x, pp->y, pp->z, n));
     if (best) *best = best_L;
+    return FSS_OK;
+}
+
+// ------------------------------------------------------------------
+// accuracy-constrained autotuner
+// ------------------------------------------------------------------
+// The treecode's cost and error both depend on (degree, theta, depth,
+// many_count, near tile); the fastest point that still meets an accuracy
+// bar is found by measuring: every candidate is built, timed and compared
+// with the FP64 direct sum at sampled targets.
+
+__global__ void k_take3(const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
+                        const int64_t *__restrict__ idx, int64_t m, double *o)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int64_t j = idx[i];
+    o[i] = u[j];
+    o[m + i] = v[j];
+    o[2 * m + i] = w[j];
+}
+
+fss_status autotune(fss_ctx *ctx, const fss_particles *pp, const fss_tune_point *cand, int ncand,
+                    const int64_t *idx, int64_t m, double max_rel_l2, int reps, int *best, double *ms,
+                    double *err)
+{
+    if (ncand < 1 || m < 1 || reps < 1 || !(max_rel_l2 > 0))
+        return set_error("autotune: need candidates, targets, reps >= 1 and a positive bar"), FSS_ERR_INVALID;
+    for (int k = 0; k < ncand; k++) {
+        const fss_tune_point &c = cand[k];
+        if (c.degree < 1 || c.degree > FSS_MAX_DEGREE || !(c.theta > 0) || c.tree_levels < 1 ||
+            c.tree_levels > 13 || c.many_count < 0 || (c.near_tile != 0 && c.near_tile != 2 && c.near_tile != 4))
+            return set_error("autotune: candidate " + std::to_string(k) + " is out of range"), FSS_ERR_INVALID;
+    }
+    hipStream_t s = ctx->stream;
+    const int64_t n = pp->n;
+    // FP64 direct sum at the targets: the reference of every candidate
+    DevBuf ref, out, take;
+    FSS_TRY(ref.ensure(sizeof(double) * 4 * m));
+    FSS_TRY(out.ensure(sizeof(double) * 4 * n));
+    FSS_TRY(take.ensure(sizeof(double) * 3 * m));
+    double *R = ref.as<double>();
+    fss_velocity rv{R, R + m, R + 2 * m, R + 3 * m};
+    FSS_TRY(velocity_direct(ctx, pp, idx, m, &rv, true));
+    std::vector<double> href(3 * m), hgot(3 * m);
+    FSS_HIP(hipMemcpyAsync(href.data(), R, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, s));
+    FSS_HIP(hipStreamSynchronize(s));
+    double rnorm = 0;
+    for (double a : href) rnorm += a * a;
+    rnorm = std::sqrt(rnorm);
+    double *O = out.as<double>();
+    fss_velocity v{O, O + n, O + 2 * n, O + 3 * n};
+    EventPair ev;
+    FSS_HIP(hipEventCreate(&ev.a));
+    FSS_HIP(hipEventCreate(&ev.b));
+    const fss_config keep = ctx->cfg;
+    const int keep_r = ctx->near_r;
+    int pick = -1, most_accurate = 0;
+    double pick_ms = std::numeric_limits<double>::infinity(), best_err = std::numeric_limits<double>::infinity();
+    auto configure = [&](const fss_tune_point &c) -> fss_status {
+        const bool new_degree = c.degree != ctx->cfg.degree;
+        ctx->cfg.degree = c.degree;
+        ctx->cfg.theta = c.theta;
+        ctx->cfg.tree_levels = c.tree_levels;
+        ctx->cfg.many_count = c.many_count;
+        ctx->near_r = c.near_tile;
+        if (new_degree) FSS_TRY(interp_setup(ctx));
+        return tree_build(ctx, pp->x, pp->y, pp->z, n);
+    };
+    for (int k = 0; k < ncand; k++) {
+        fss_status st = configure(cand[k]);
+        if (st != FSS_OK) {
+            ctx->cfg = keep;
+            ctx->near_r = keep_r;
+            (void)interp_setup(ctx);
+            return st;
+        }
+        FSS_TRY(velocity_tree(ctx, pp, &v)); // warm-up (workspace growth)
+        FSS_HIP(hipEventRecord(ev.a, s));
+        for (int r = 0; r < reps; r++) FSS_TRY(velocity_tree(ctx, pp, &v));
+        FSS_HIP(hipEventRecord(ev.b, s));
+        hipLaunchKernelGGL(k_take3, dim3((unsigned)div_up(m, 256)), dim3(256), 0, s, v.u, v.v, v.w, idx, m,
+                           take.as<double>());
+        FSS_HIP(hipGetLastError());
+        FSS_HIP(hipMemcpyAsync(hgot.data(), take.p, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, s));
+        FSS_HIP(hipStreamSynchronize(s));
+        float t = 0;
+        FSS_HIP(hipEventElapsedTime(&t, ev.a, ev.b));
+        const double tm = double(t) / reps;
+        double dn = 0;
+        for (int64_t i = 0; i < 3 * m; i++) {
+            const double d = hgot[i] - href[i];
+            dn += d * d;
+        }
+        const double e = std::sqrt(dn) / rnorm;
+        if (ms) ms[k] = tm;
+        if (err) err[k] = e;
+        if (e <= max_rel_l2 && tm < pick_ms) {
+            pick_ms = tm;
+            pick = k;
+        }
+        if (e < best_err) {
+            best_err = e;
+            most_accurate = k;
+        }
+    }
+    const int use = pick >= 0 ? pick : most_accurate;
+    if (use != ncand - 1) FSS_TRY(configure(cand[use]));
+    if (best) *best = pick;
     return FSS_OK;
 }
 

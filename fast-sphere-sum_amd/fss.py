@@ -52,6 +52,11 @@ class Velocity(C.Structure):
     _fields_ = [("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p), ("dvort", C.c_void_p)]
 
 
+class TunePoint(C.Structure):
+    _fields_ = [("degree", C.c_int), ("theta", C.c_double), ("tree_levels", C.c_int), ("many_count", C.c_int),
+                ("near_tile", C.c_int)]
+
+
 class TreeStats(C.Structure):
     _fields_ = [("n", C.c_int64), ("tree_levels", C.c_int), ("n_nodes", C.c_int64),
                 ("n_interactions", C.c_int64), ("n_type", C.c_int64 * 4),
@@ -77,6 +82,8 @@ SIGNATURES = {
     "fss_batch_build": (_int, [_vp, _int, _vp, _vp, _vp, _vp, _vp]),
     "fss_autotune_levels": (_int, [_vp, C.POINTER(Particles), _int, _int, _int, C.POINTER(_int), _vp]),
     "fss_levels_for": (_int, [C.c_int64, _dbl]),
+    "fss_autotune": (_int, [_vp, C.POINTER(Particles), C.POINTER(TunePoint), _int, _vp, _i64, _dbl, _int,
+                            C.POINTER(_int), _vp, _vp]),
     "fss_set_near_tile": (_int, [_vp, _int]),
     "fss_tree_export": (_int, [_vp] + [_vp] * 8),
     "fss_velocity_direct": (_int, [_vp, C.POINTER(Particles), C.POINTER(Velocity)]),
@@ -359,6 +366,30 @@ class Context:
         self.n = x.numel()
         self.cfg.tree_levels = best.value
         return best.value, {lmin + k: float(v) for k, v in enumerate(ms)}
+
+    def autotune(self, x, y, z, vort, area, candidates, idx, max_rel_l2, reps=3):
+        """Fastest of `candidates` (dicts with degree, theta, tree_levels,
+        many_count, near_tile) whose rel-L2 against the FP64 direct sum at
+        the targets idx is <= max_rel_l2; the context keeps it.  Returns
+        (index or -1, per-candidate ms, per-candidate errors)."""
+        import torch
+        p = self._particles(x, y, z, vort, area)
+        arr = (TunePoint * len(candidates))()
+        for k, c in enumerate(candidates):
+            arr[k] = TunePoint(int(c["degree"]), float(c["theta"]), int(c["tree_levels"]),
+                               int(c["many_count"]), int(c.get("near_tile", 0)))
+        idx = idx.to(device=self.device, dtype=torch.int64).contiguous()
+        ms = np.zeros(len(candidates))
+        err = np.zeros(len(candidates))
+        best = C.c_int(-2)
+        _check(lib().fss_autotune(self.h, C.byref(p), arr, len(candidates), _dev_ptr(idx), idx.numel(),
+                                  float(max_rel_l2), int(reps), C.byref(best), _np_ptr(ms), _np_ptr(err)))
+        b = best.value
+        c = candidates[b if b >= 0 else int(np.argmin(err))]
+        self.cfg.degree, self.cfg.theta = int(c["degree"]), float(c["theta"])
+        self.cfg.tree_levels, self.cfg.many_count = int(c["tree_levels"]), int(c["many_count"])
+        self.n = x.numel()
+        return b, ms, err
 
     def set_near_tile(self, targets_per_lane):
         _check(lib().fss_set_near_tile(self.h, int(targets_per_lane)))

@@ -151,6 +151,26 @@ fss_status fss_batch_build(fss_ctx *ctx, int ninst, const int64_t *n, const int 
  * time of every depth. */
 fss_status fss_autotune_levels(fss_ctx *ctx, const fss_particles *p, int lmin, int lmax, int reps,
                                int *best_levels, double *ms);
+/* Accuracy-constrained autotuner (no reference counterpart: the reference
+ * takes degree, theta, depth and many_count by hand, run_config.h).  For every
+ * candidate the tree is built on p, `reps` stages are timed after a warm-up
+ * (HIP events) and the velocity at the m targets idx (device int64) is
+ * compared with the FP64 direct sum there (rel-L2 over the three
+ * components).  The fastest candidate with error <= max_rel_l2 is kept: the
+ * context's config, near tile and tree are its own afterwards and *best is
+ * its index; when none meets the bar *best = -1 and the most accurate one is
+ * kept.  ms and err (host, ncand entries each, may be NULL) receive every
+ * candidate's stage time and error. */
+typedef struct {
+    int degree;
+    double theta;
+    int tree_levels;
+    int many_count;
+    int near_tile; /* targets per lane of the near kernel: 2, 4, or 0 (default) */
+} fss_tune_point;
+fss_status fss_autotune(fss_ctx *ctx, const fss_particles *p, const fss_tune_point *cand, int ncand,
+                        const int64_t *idx, int64_t m, double max_rel_l2, int reps, int *best, double *ms,
+                        double *err);
 /* Depth whose leaves hold `leaf_occupancy` particles on average for n
  * particles: round(1 + log4(n / (20 occupancy))), clamped to [1, 13]. */
 int fss_levels_for(int64_t n, double leaf_occupancy);

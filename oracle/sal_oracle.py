@@ -36,9 +36,11 @@ def coefficients(h, k, rho_ratio=RHO_RATIO, cesaro=False):
     return c
 
 
-def legendre_series(c, t):
-    """sum_n c_n P_n(t) by the three-term recurrence (P_0 = 1, P_1 = t)."""
-    t = np.asarray(t, dtype=np.float64)
+def legendre_series(c, t, dtype=np.float64):
+    """sum_n c_n P_n(t) by the three-term recurrence (P_0 = 1, P_1 = t), in
+    `dtype` (np.longdouble gives a reference with 11 more bits)."""
+    t = np.asarray(t, dtype=dtype)
+    c = np.asarray(c, dtype=dtype)
     p0 = np.ones_like(t)
     p1 = t.copy()
     acc = c[0] * p0 + (c[1] * p1 if len(c) > 1 else 0)
