@@ -940,9 +940,12 @@ __device__ inline void add_interp(const double *__restrict__ tinv, const double 
 // interpolants of the nodes on that path, at the position held by the record;
 // the rare memberships outside the main path come from the per-level maps.
 // The sums, with the rhs_func scaling (rhs_eval.h:54-55), go straight to the
-// particle's place in the caller's arrays.
+// particle's place in the caller's arrays.  No launch bounds: under the
+// default cap of 128 VGPRs (a few spilled at degree 6-8) four waves fit per
+// SIMD, which measured faster than 159 VGPRs and three (degree 8, level-10
+// grid: 1.23 against 1.47 ms).
 template <typename Real, int D>
-__global__ void __launch_bounds__(256) k_combine(const int32_t *__restrict__ recid, const int32_t *__restrict__ recnode,
+__global__ void k_combine(const int32_t *__restrict__ recid, const int32_t *__restrict__ recnode,
                           const uint8_t *__restrict__ rinfo, const int4 *__restrict__ nrange,
                           const int32_t *__restrict__ oslot, const int32_t *__restrict__ node_slot,
                           const int32_t *__restrict__ nodeof, const int32_t *__restrict__ inv_active,

@@ -318,7 +318,7 @@ __device__ inline double sal_interp(const double *__restrict__ tinv, const doubl
 // per main record (tree order): near slots of its main path + interpolants of
 // the nodes on it; memberships outside the main path from the per-level maps
 template <int D>
-__global__ void __launch_bounds__(256) k_sal_combine(const int32_t *__restrict__ recid, const int32_t *__restrict__ recnode,
+__global__ void k_sal_combine(const int32_t *__restrict__ recid, const int32_t *__restrict__ recnode,
                               const uint8_t *__restrict__ rinfo, const int4 *__restrict__ nrange,
                               const int32_t *__restrict__ oslot, const int32_t *__restrict__ node_slot,
                               const int32_t *__restrict__ nodeof, const int32_t *__restrict__ inv_active,

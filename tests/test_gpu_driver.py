@@ -98,21 +98,27 @@ def test_driver_reads_reference_input_files(tmp_path):
     np.testing.assert_allclose(a, b, rtol=0, atol=1e-13)
 
 
-@pytest.mark.parametrize("device", [False, True])
-def test_driver_amr_step_matches_oracle(orc, tmp_path, device):
+@pytest.mark.parametrize("device,fast", [(False, False), (True, False), (False, True), (True, True)])
+def test_driver_amr_step_matches_oracle(orc, tmp_path, device, fast):
     """-a -b 1 (new_dynamics.cpp:218-225): after the step the mesh is refined
     by the oracle's amr restatement on the stepped state, then projected.  The
     host path refines the unprojected step as the reference does; the device
-    step (-g) projects inside its fused update."""
+    step (-g) projects inside its fused update.  With -f the step runs on the
+    treecode and the tree is rebuilt on the refined points after the step (the
+    reference reassigns into arrays sized for the initial count instead:
+    new_dynamics.cpp:79-83 with icos_utils.h:164, an out-of-bounds write once
+    AMR has added points)."""
     import oracle
     import fss
-    args = ["-x", "-c", "3", "-i", "gv", "-a", "-b", "1"] + (["-g"] if device else [])
+    args = ["-x", "-c", "3", "-i", "gv", "-a", "-b", "1"] + (["-g"] if device else []) \
+        + (["-f", "-l", "3"] if fast else [])
     data, name = _run(tmp_path, args)
     n = 10 * 4 ** 3 + 2
-    assert "_amr_" in name
+    assert "_amr_" in name and (("_fast_3_" in name) == fast)
     s0 = data[:n]
     x, y, z, vort, tracer, area = s0.T
-    sx, sy, sz, sv = orc.rk4_step(x, y, z, vort, area, 0.01, project=device)
+    tree = orc.Tree(x, y, z, 3, theta=0.7, many_count=10, degree=2) if fast else None
+    sx, sy, sz, sv = orc.rk4_step(x, y, z, vort, area, 0.01, tree=tree, project=device)
     _, _, _, tris = fss.grid_icosahedral(3)
     T = len(tris)
     M = 10 * 4 ** 4 + 2
