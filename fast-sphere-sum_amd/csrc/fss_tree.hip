@@ -264,6 +264,59 @@ __global__ void k_nrange_counts(const int32_t *__restrict__ counts, const int32_
 // ------------------------------------------------------------------
 enum : int8_t { ACT_SKIP = -1, ACT_SPLIT_T = 4, ACT_SPLIT_S = 5 };
 
+// number of interactions of each type (pp, pc, cp, cc): per-block counts in
+// LDS, then one global atomic per type and block (integer sums: the result
+// does not depend on the order)
+__global__ void __launch_bounds__(256) k_type_counts(const int32_t *__restrict__ inter, int64_t n,
+                                                     unsigned long long *cnt)
+{
+    __shared__ unsigned int c[4];
+    if (threadIdx.x < 4) c[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&c[inter[7 * i + 6] & 3], 1u);
+    __syncthreads();
+    if (threadIdx.x < 4 && c[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], (unsigned long long)c[threadIdx.x]);
+}
+
+// Proxy items: the class-B ranges of every interpolant slot cut into parts of
+// about FSS_PROXY_CHUNK sources, in range order (a part closes at the range
+// that takes its count to the chunk size, or at the slot's last range).  One
+// thread per slot, counting first and writing after the scan of the counts.
+__device__ inline int32_t chunk_parts(const ProxyItem &it, const Range *__restrict__ rb, ProxyItem *out,
+                                      int32_t slot)
+{
+    int32_t np = 0, b = it.b_beg;
+    int64_t acc = 0;
+    for (int32_t r = it.b_beg; r < it.b_end; r++) {
+        acc += rb[r].count & ~FSS_SELF_BIT;
+        if (acc >= FSS_PROXY_CHUNK || r + 1 == it.b_end) {
+            if (out) out[np] = ProxyItem{it.node, b, r + 1, slot};
+            np++;
+            b = r + 1;
+            acc = 0;
+        }
+    }
+    return np;
+}
+
+__global__ void k_chunk_count(const ProxyItem *__restrict__ one, int64_t nslot, const Range *__restrict__ rb,
+                              int32_t *cnt)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nslot) cnt[q] = chunk_parts(one[q], rb, nullptr, (int32_t)q);
+}
+
+__global__ void k_chunk_fill(const ProxyItem *__restrict__ one, int64_t nslot, const Range *__restrict__ rb,
+                             const int32_t *__restrict__ pos, const int32_t *__restrict__ cnt,
+                             ProxyItem *parts, ProxySlot *slots)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nslot) return;
+    chunk_parts(one[q], rb, parts + pos[q], (int32_t)q);
+    slots[q] = ProxySlot{one[q].node, pos[q], pos[q] + cnt[q], 0};
+}
+
 __global__ void k_trav_init(int4 *F)
 {
     int i = threadIdx.x + blockIdx.x * blockDim.x;
@@ -929,9 +982,16 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
 
     // type counts
     if (ninter > 0) {
-        std::vector<int32_t> h(7 * ninter);
-        FSS_TRY(d2h(h.data(), T.inter.p, sizeof(int32_t) * 7 * ninter, s));
-        for (int64_t i = 0; i < ninter; i++) T.stats.n_type[h[7 * i + 6]]++;
+        DevBuf tc;
+        FSS_TRY(tc.ensure(sizeof(unsigned long long) * 4));
+        FSS_HIP(hipMemsetAsync(tc.p, 0, sizeof(unsigned long long) * 4, s));
+        const int64_t nb = std::min<int64_t>(div_up(ninter, 256), 4096);
+        hipLaunchKernelGGL(k_type_counts, dim3((unsigned)nb), dim3(256), 0, s, T.inter.as<int32_t>(), ninter,
+                           tc.as<unsigned long long>());
+        FSS_HIP(hipGetLastError());
+        unsigned long long h[4];
+        FSS_TRY(d2h(h, tc.p, sizeof(h), s));
+        for (int t = 0; t < 4; t++) T.stats.n_type[t] = (int64_t)h[t];
     }
 
     // ---- evaluation plan ----
@@ -1038,40 +1098,29 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
                        hasB.as<int32_t>(), nn, nA, T.near_items.as<NearItem>(),
                        prox_one.as<ProxyItem>(), T.node_slot.as<int32_t>());
     // split long class-B lists into chunks of about FSS_PROXY_CHUNK sources
-    // (host pass over the per-node lists; order inside a node is kept)
+    // (order inside a node is kept)
     {
-        std::vector<ProxyItem> one(T.n_slot);
-        std::vector<Range> rb(T.n_rb);
-        if (T.n_slot > 0) FSS_TRY(d2h(one.data(), prox_one.p, sizeof(ProxyItem) * T.n_slot, s));
-        if (T.n_rb > 0) FSS_TRY(d2h(rb.data(), T.ranges_b.p, sizeof(Range) * T.n_rb, s));
-        std::vector<ProxyItem> parts;
-        std::vector<ProxySlot> slots(T.n_slot);
-        for (int64_t q = 0; q < T.n_slot; q++) {
-            const ProxyItem &it = one[q];
-            slots[q].node = it.node;
-            slots[q].part_beg = (int32_t)parts.size();
-            slots[q].pad = 0;
-            int32_t b = it.b_beg;
-            int64_t acc = 0;
-            for (int32_t r = it.b_beg; r < it.b_end; r++) {
-                acc += rb[r].count & ~FSS_SELF_BIT;
-                if (acc >= FSS_PROXY_CHUNK || r + 1 == it.b_end) {
-                    parts.push_back(ProxyItem{it.node, b, r + 1, (int32_t)q});
-                    b = r + 1;
-                    acc = 0;
-                }
-            }
-            slots[q].part_end = (int32_t)parts.size();
-        }
-        T.n_proxy = (int64_t)parts.size();
-        FSS_TRY(T.proxy_items.ensure(sizeof(ProxyItem) * std::max<int64_t>(1, T.n_proxy)));
         FSS_TRY(T.proxy_slots.ensure(sizeof(ProxySlot) * std::max<int64_t>(1, T.n_slot)));
-        if (T.n_proxy > 0)
-            FSS_HIP(hipMemcpyAsync(T.proxy_items.p, parts.data(), sizeof(ProxyItem) * T.n_proxy,
-                                   hipMemcpyHostToDevice, s));
+        T.n_proxy = 0;
+        DevBuf pcnt, ppos;
+        if (T.n_slot > 0) {
+            FSS_TRY(pcnt.ensure(sizeof(int32_t) * T.n_slot));
+            FSS_TRY(ppos.ensure(sizeof(int32_t) * T.n_slot));
+            const unsigned gs = (unsigned)div_up(T.n_slot, 256);
+            hipLaunchKernelGGL(k_chunk_count, dim3(gs), dim3(256), 0, s, prox_one.as<ProxyItem>(), T.n_slot,
+                               T.ranges_b.as<Range>(), pcnt.as<int32_t>());
+            FSS_TRY(cub_exclusive_sum(S.cub, pcnt.as<int32_t>(), ppos.as<int32_t>(), T.n_slot, s));
+            int32_t tail[2];
+            FSS_TRY(d2h(&tail[0], ppos.as<int32_t>() + T.n_slot - 1, 4, s));
+            FSS_TRY(d2h(&tail[1], pcnt.as<int32_t>() + T.n_slot - 1, 4, s));
+            T.n_proxy = (int64_t)tail[0] + tail[1];
+        }
+        FSS_TRY(T.proxy_items.ensure(sizeof(ProxyItem) * std::max<int64_t>(1, T.n_proxy)));
         if (T.n_slot > 0)
-            FSS_HIP(hipMemcpyAsync(T.proxy_slots.p, slots.data(), sizeof(ProxySlot) * T.n_slot,
-                                   hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_chunk_fill, dim3((unsigned)div_up(T.n_slot, 256)), dim3(256), 0, s,
+                               prox_one.as<ProxyItem>(), T.n_slot, T.ranges_b.as<Range>(), ppos.as<int32_t>(),
+                               pcnt.as<int32_t>(), T.proxy_items.as<ProxyItem>(), T.proxy_slots.as<ProxySlot>());
+        FSS_HIP(hipGetLastError());
         FSS_HIP(hipStreamSynchronize(s));
     }
     FSS_TRY(T.nodeof.ensure(sizeof(int32_t) * (size_t)L * n));
