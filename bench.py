@@ -56,16 +56,17 @@ def workload(name):
     if name == "c3":
         # the accuracy point is picked by the library's autotuner among
         # candidates on the measured cost/accuracy frontier of this workload
-        # (degree 5-8, theta 0.3-0.65, 7-8 levels, many_count 10-48, two or
-        # four targets per lane in the near kernel)
+        # (degree 5-8, theta 0.3-0.6, 8 levels, many_count 10-64, two or four
+        # targets per lane in the near kernel)
         cands = [dict(degree=5, theta=0.3, tree_levels=8, many_count=10),
-                 dict(degree=7, theta=0.47, tree_levels=8, many_count=48),
-                 dict(degree=8, theta=0.55, tree_levels=8, many_count=45),
+                 dict(degree=7, theta=0.5, tree_levels=8, many_count=64),
+                 dict(degree=7, theta=0.52, tree_levels=8, many_count=64),
+                 dict(degree=7, theta=0.52, tree_levels=8, many_count=64, near_tile=4),
+                 dict(degree=7, theta=0.54, tree_levels=8, many_count=64),
                  dict(degree=8, theta=0.58, tree_levels=8, many_count=45),
-                 dict(degree=8, theta=0.58, tree_levels=8, many_count=45, near_tile=4),
-                 dict(degree=8, theta=0.6, tree_levels=7, many_count=45),
-                 dict(degree=8, theta=0.62, tree_levels=8, many_count=45),
-                 dict(degree=8, theta=0.65, tree_levels=8, many_count=45)]
+                 dict(degree=8, theta=0.58, tree_levels=8, many_count=64),
+                 dict(degree=8, theta=0.58, tree_levels=8, many_count=64, near_tile=4),
+                 dict(degree=8, theta=0.6, tree_levels=8, many_count=64)]
         return dict(name="c3", level=10, field="gauss", targets=10000, seed=3033, headline="mixed",
                     config=dict(tree_levels=8, degree=3, theta=0.6, many_count=10),
                     metric=dict(tree_levels=8, degree=5, theta=0.3, many_count=10), candidates=cands)
@@ -134,6 +135,17 @@ def time_stage(torch, ctx, call, steps, warmup):
     calls, _, near_ms = ctx.timing_mean()
     assert calls == steps
     return t, near_ms, ctx.timing_far_mean()
+
+
+def calibration_copy(torch):
+    """One plain device copy of 512 MiB (torch clone): the FETCH_SIZE and
+    WRITE_SIZE counters of this dispatch scale those of the kernels."""
+    buf = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda:0").fill_(1.0)
+    torch.cuda.synchronize()
+    cp = buf.clone()  # reads 512 MiB, writes 512 MiB
+    torch.cuda.synchronize()
+    log(f"calibration copy: {buf.numel() * 8} bytes read and written")
+    del buf, cp
 
 
 def self_pairs(ctx):
@@ -222,25 +234,38 @@ def run_batch(args, W, fss, torch):
                            many_count=W["many_count"], precision=precision, eval_mode=fss.EVAL_BLTC,
                            assign_mode=fss.ASSIGN_ROBUST, node_rule=fss.NODES_SIN2)
 
-    # autotune the leaf size on the instance that holds the median particle
-    # (the batch's work sits in its large instances; a small one is bound by
-    # launch latency and favours small leaves), transfer it by occupancy
-    km = int(np.argsort(sizes)[K // 2])
-    nk = int(sizes[km])
+    # autotune depth and near tile on the instance that holds the median
+    # particle (the batch's work sits in its large instances; a small one is
+    # bound by launch latency and favours small leaves), transfer the depth by
+    # leaf occupancy; every candidate is timed and its error recorded (the
+    # workload's degree 2 sets the accuracy, so no bar applies)
     order = np.argsort(sizes)
     kt = int(order[np.searchsorted(np.cumsum(sizes[order]), sizes.sum() / 2)])
     nt = int(sizes[kt])
     sl = slice(int(base[kt]), int(base[kt + 1]))
+    cands = [dict(degree=W["degree"], theta=W["theta"], tree_levels=L, many_count=W["many_count"], near_tile=r)
+             for L in range(4, 9) for r in (2, 4)]
+    if args.point:
+        # fixed depth and tile (profiling runs): 'degree,theta,levels,many_count,tile',
+        # only the depth and the tile are used
+        f = args.point.split(",")
+        cands = [dict(cands[0], tree_levels=int(f[2]), near_tile=int(f[4]) if len(f) > 4 else 2)]
     tctx = make_ctx(fss.MIXED)
-    best, tms = tctx.autotune_levels(*[d[sl] for d in dev], 3, 8, reps=3)
+    tidx = torch.from_numpy(fss.sample_targets(nt, min(2000, nt), W["seed"] + 300))
+    bi, tms, terr = tctx.autotune(*[d[sl] for d in dev], cands, tidx, 1.0, reps=3)
+    best, tile = cands[bi]["tree_levels"], cands[bi]["near_tile"]
     occ = nt / (20 * 4 ** (best - 1))
     levels = [fss.levels_for(int(n), occ) for n in sizes]
-    log(f"autotune on N={nt}: ms per depth {tms} -> {best} levels, leaf occupancy {occ:.1f}")
+    log(f"autotune on N={nt}: " + ", ".join(f"L{c['tree_levels']}/R{c['near_tile']} {m:.3f} ms"
+                                              for c, m in zip(cands, tms))
+        + f" -> {best} levels, {tile} targets per lane, leaf occupancy {occ:.1f}")
     del tctx
 
     ctxs = {}
-    for prec in (fss.MIXED, fss.FP64):
+    for prec in ((fss.MIXED,) if args.profile else (fss.MIXED, fss.FP64)):
         c = make_ctx(prec)
+        if prec == fss.MIXED:
+            c.set_near_tile(tile)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         c.batch_build(*dev[:3], sizes, levels)
@@ -250,6 +275,9 @@ def run_batch(args, W, fss, torch):
     out = [torch.empty(NT, dtype=torch.float64, device="cuda:0") for _ in range(4)]
     t_stage, near_ms, far_ms = time_stage(torch, ctx, lambda: ctx.velocity_tree(*dev, out=out), args.steps,
                                   args.warmup)
+    if args.profile:
+        calibration_copy(torch)
+        return
     um = [o.cpu().numpy() for o in out]
     c64 = ctxs[fss.FP64][0]
     t_stage64, _, _ = time_stage(torch, c64, lambda: c64.velocity_tree(*dev, out=out), max(3, args.steps // 4),
@@ -286,25 +314,27 @@ def run_batch(args, W, fss, torch):
     # assignment): NT masked self pairs are not algorithmic work
     alg_pairs = st.pairs_near - NT
     achieved = alg_pairs * FLOPS_PAIR_FP32 / (near_ms * 1e-3) / 1e12
-    roofline = dict(kernel="k_near<float,2>", bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS,
+    roofline = dict(kernel=f"k_near<float,{tile},false>", bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS,
                     unit="TFLOP/s", frac=achieved / PEAK_FP32_TFLOPS,
                     traffic=traffic_from_profiles("k_near32", "c4"), flops_per_launch=alg_pairs * FLOPS_PAIR_FP32,
                     note=f"{alg_pairs} near pairs x {FLOPS_PAIR_FP32} flops / mean "
-                         "k_near<float,2> duration (HIP events on the context stream)")
+                         f"k_near<float,{tile},false> duration (HIP events on the context stream); traffic: "
+                         "HBM bytes per launch from profiles/k_near32_c4_traffic.md")
     cpu = None
     if not args.no_cpu:
-        # the oracle's treecode stage (same BLTC parameters) on the median instance
+        # the oracle's treecode stage (same BLTC parameters) on the tuning instance
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         cores = host_cores()
-        x, y, z, v, a = parts[km]
-        T = oracle.Tree(x, y, z, levels[km], theta=W["theta"], many_count=W["many_count"],
+        x, y, z, v, a = parts[kt]
+        T = oracle.Tree(x, y, z, levels[kt], theta=W["theta"], many_count=W["many_count"],
                         degree=W["degree"], assign=oracle.ASSIGN_ROBUST, eval_mode=oracle.EVAL_BLTC, node_rule=1)
         t0 = time.perf_counter()
         T.rhs(v, a, nthreads=cores)
         t = time.perf_counter() - t0
-        cpu = dict(value=nk / t, unit="velocities/s", cores=cores, kind="port",
-                   sample=f"one treecode stage of the median instance (N={nk}, {levels[km]} levels, oracle "
+        cpu = dict(value=nt / t, unit="velocities/s", cores=cores, kind="port",
+                   sample=f"one treecode stage of the instance holding the median particle (N={nt}, "
+                          f"{levels[kt]} levels, oracle "
                           f"BLTC) {t:.2f} s on {cores} threads of {cpu_model()}", seconds=t,
                    cpu_model=cpu_model())
         log(f"cpu baseline: {cpu['value']:.3e} vel/s ({cpu['sample']})")
@@ -318,14 +348,17 @@ def run_batch(args, W, fss, torch):
                                 f"vortices, degree {W['degree']}, theta {W['theta']}, mixed precision, "
                                 f"{W['rk4_steps']} RK4 steps dt={W['dt']} from a hipGraph"),
                    "n_total": NT, "instances": K, "seed": W["seed"], "degree": W["degree"],
-                   "theta": W["theta"], "tuned_levels": best, "leaf_occupancy": occ,
+                   "theta": W["theta"], "tuned_levels": best, "near_tile": tile, "leaf_occupancy": occ,
                    "levels_hist": {int(l): int(levels.count(l)) for l in sorted(set(levels))},
                    "parallelism": "1 GPU"},
         "accuracy": {"rel_l2_vs_fp64_direct": l2, "max_rel_vs_fp64_direct": mx,
                      "rel_l2_mixed_vs_fp64_bltc": l2m, "targets": per * K},
         "stage_ms_mixed": t_stage * 1e3, "stage_ms_fp64": t_stage64 * 1e3,
         "speedup_mixed_over_fp64": t_stage64 / t_stage, "tree_build_ms": build_ms,
-        "autotune_ms": tms, "roofline": roofline, "cpu_baseline": cpu,
+        "autotune": {"instance_n": nt, "candidates": [dict(c, ms=float(m), rel_l2=float(e))
+                                                      for c, m, e in zip(cands, tms, terr)],
+                     "chosen": bi},
+        "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
@@ -560,12 +593,7 @@ def run_single(args, W, fss, torch):
                 del c
 
     if args.profile:
-        buf = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda:0").fill_(1.0)
-        torch.cuda.synchronize()
-        cp = buf.clone()  # reads 512 MiB, writes 512 MiB
-        torch.cuda.synchronize()
-        log(f"calibration copy: {buf.numel() * 8} bytes read and written")
-        del buf, cp
+        calibration_copy(torch)
 
     cpu = None
     if not args.no_cpu:

@@ -36,6 +36,13 @@ def test_driver_step_matches_oracle(orc, tmp_path, args):
     assert name.startswith(f"output_{n}_")
     s0, s1 = data[:n], data[n:]
     x, y, z, vort, tracer, area = s0.T
+    if "gv" in args:
+        # gauss_vortex (vorticity_functions.h:26-46): 4 pi exp(-16 |c - x|^2)
+        # about latitude pi/20, longitude 0, area-weighted mean removed
+        c = np.array([np.cos(np.pi / 20), 0.0, np.sin(np.pi / 20)])
+        g = 4 * np.pi * np.exp(-16 * ((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2))
+        np.testing.assert_allclose(vort, g - np.dot(g, area) / (4 * np.pi), rtol=0, atol=1e-13)
+        np.testing.assert_allclose(tracer, 2 + np.arcsin(z), rtol=0, atol=1e-13)
     tree = orc.Tree(x, y, z, 3, theta=0.7, many_count=10, degree=2) if "-f" in args else None
     ref = orc.rk4_step(x, y, z, vort, area, 0.01, tree=tree)
     for c in range(4):
