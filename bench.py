@@ -61,8 +61,9 @@ def workload(name):
         cands = [dict(degree=5, theta=0.3, tree_levels=8, many_count=10),
                  dict(degree=7, theta=0.5, tree_levels=8, many_count=64),
                  dict(degree=7, theta=0.52, tree_levels=8, many_count=64),
-                 dict(degree=7, theta=0.52, tree_levels=8, many_count=64, near_tile=4),
-                 dict(degree=7, theta=0.54, tree_levels=8, many_count=64),
+                 dict(degree=7, theta=0.525, tree_levels=8, many_count=64),
+                 dict(degree=7, theta=0.525, tree_levels=8, many_count=64, near_tile=4),
+                 dict(degree=7, theta=0.53, tree_levels=8, many_count=64),
                  dict(degree=8, theta=0.58, tree_levels=8, many_count=45),
                  dict(degree=8, theta=0.58, tree_levels=8, many_count=64),
                  dict(degree=8, theta=0.58, tree_levels=8, many_count=64, near_tile=4),

@@ -2,8 +2,8 @@
 // set of launches.
 //
 // Each instance's tree, lists and evaluation plan are built as for a single
-// problem (fss_tree.hip) on its slice of the concatenated particle arrays;
-// the plans are then merged into one flattened plan: node indices, source
+// problem (fss_tree.hip) on its slice of the concatenated particle arrays
+// and appended on the device to one flattened plan: node indices, source
 // positions, range lists, proxy chunks and interpolant slots are offset per
 // instance so that every per-stage kernel (gather, upward, proxy, near,
 // solve, combine) covers all instances in a single launch, and the RK4 step

@@ -77,16 +77,6 @@ __device__ inline double dot_ref(const double4 &a, const double4 &b)
 // FP32: records are offsets from a common node centre and carry the doubled
 // weight 2 w; 2 (1 - x.y) = |d|^2 + off2, where off2 = 1 - |y|^2 for a target
 // off the sphere (planar proxy), so c = 2 w / (|d|^2 + off2) = w / (1 - x.y).
-__device__ inline void pair_eval(const float4 &xt, const float4 &s, float off2, float &S0, float &S1,
-                                 float &S2)
-{
-    float dx = s.x - xt.x, dy = s.y - xt.y, dz = s.z - xt.z;
-    float den = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, off2)));
-    float c = s.w * rcp32(den);
-    S0 = fmaf(c, dx, S0);
-    S1 = fmaf(c, dy, S1);
-    S2 = fmaf(c, dz, S2);
-}
 
 // The same pair with the self pair masked out (ranges that hold the target
 // itself): the weight is replaced by 0 after the division, so the self pair
@@ -114,13 +104,6 @@ __device__ inline void near_pair_masked(const float4 &xt, const float4 &s, bool 
     S0 = fmaf(c, dx, S0);
     S1 = fmaf(c, dy, S1);
     S2 = fmaf(c, dz, S2);
-}
-
-// FP32 far field: the same offset form
-__device__ inline void pair_far(const float4 &xt, const float4 &s, float off, float &S0, float &S1,
-                                float &S2)
-{
-    pair_eval(xt, s, off, S0, S1, S2);
 }
 
 // A record moved to the frame of a target node with centre ct: FP64 records
@@ -672,9 +655,12 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
 // ------------------------------------------------------------------
 // proxy values (class B): p proxies of a node against one chunk of sources
 // ------------------------------------------------------------------
-// One source against the R proxies of a lane.  FP64: one reciprocal for the R
+// One source against the R proxies of a lane: one reciprocal for the R
 // denominators as in near_tile (far pairs are separated by the acceptance
-// criterion, so no denominator vanishes).  FP32: one pair at a time.
+// criterion, so no denominator vanishes).  FP32 works on the offset form
+// (2 (1 - x.y) = |d|^2 + off2); the shared reciprocal took the
+// degree-7 proxy kernel of the level-10 grid from 3.93-4.00 to 3.49-3.50 ms
+// (three proxies per lane; one per pair before).
 template <int R, typename AccT>
 __device__ inline void far_tile(const double4 *yk, const double4 &s, const double *, AccT &A)
 {
@@ -702,10 +688,28 @@ __device__ inline void far_tile(const double4 *yk, const double4 &s, const doubl
 template <int R, typename AccT>
 __device__ inline void far_tile(const float4 *yk, const float4 &s, const float *at, AccT &A)
 {
+    float d[R][3], den[R], pre[R], c[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        d[q][0] = s.x - yk[q].x;
+        d[q][1] = s.y - yk[q].y;
+        d[q][2] = s.z - yk[q].z;
+        den[q] = fmaf(d[q][0], d[q][0], fmaf(d[q][1], d[q][1], fmaf(d[q][2], d[q][2], at[q])));
+        pre[q] = q == 0 ? den[0] : pre[q - 1] * den[q];
+    }
+    float rw = s.w * rcp32(pre[R - 1]);
+#pragma unroll
+    for (int q = R - 1; q > 0; q--) {
+        c[q] = pre[q - 1] * rw;
+        rw = den[q] * rw;
+    }
+    c[0] = rw;
 #pragma unroll
     for (int q = 0; q < R; q++) {
         float *pq = A.part(q);
-        pair_far(yk[q], s, at[q], pq[0], pq[1], pq[2]);
+        pq[0] = fmaf(c[q], d[q][0], pq[0]);
+        pq[1] = fmaf(c[q], d[q][1], pq[1]);
+        pq[2] = fmaf(c[q], d[q][2], pq[2]);
     }
 }
 
