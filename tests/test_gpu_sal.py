@@ -108,7 +108,9 @@ def test_learned_kernel_trains_and_gains(fsslib, sal):
     l0 = ctx.sal_mlp_train(1, samples=1 << 15, lr=3e-3, s_min=1e-3, seed=2)
     loss = ctx.sal_mlp_train(3000, samples=1 << 15, lr=3e-3, s_min=1e-3, seed=2)
     print("loss first/last", l0, loss, flush=True)
-    assert loss < 0.1 * l0
+    # training is deterministic (fixed-order gradient sums): measured 1.467 ->
+    # 0.0326, a ratio of 0.022
+    assert loss < 0.03 * l0
     s = np.geomspace(2e-3, 1, 500)
     ts = torch.from_numpy(s)
     g_ser = ctx.sal_kernel(ts, kind=0).cpu().numpy()
@@ -120,9 +122,9 @@ def test_learned_kernel_trains_and_gains(fsslib, sal):
     e_sing = np.linalg.norm(g_sing - g_ser) / np.linalg.norm(g_ser)
     print("kernel rel-L2 learned", e_ml, "singular part only", e_sing)
     # the remainder keeps the truncated series' Gibbs ripple at small s, which
-    # a 64-wide network follows only in part (a torch fp32 fit of the same
-    # network reaches e_ml / e_sing = 0.43 on this set)
-    assert e_ml < 0.6 * e_sing
+    # a 64-wide network follows only in part: measured e_ml / e_sing = 0.502
+    # (0.0616 / 0.1227); a torch fp32 fit of the same network reaches 0.43
+    assert e_ml < 0.55 * e_sing
     # the convolution
     x, y, z, _, area, _ = icos_case(fsslib, 5)
     eta = np.exp(-((x - 0.5) ** 2 + (y - 0.3) ** 2 + (z - 0.8) ** 2) / 0.05)
@@ -136,7 +138,8 @@ def test_learned_kernel_trains_and_gains(fsslib, sal):
     sing = ctx.sal_direct(*d, idx=torch.from_numpy(idx), kind=1).cpu().numpy()
     e_conv_sing = np.linalg.norm(sing - ref) / np.linalg.norm(ref)
     print("SAL rel-L2 learned", e_conv, "singular part only", e_conv_sing)
-    assert e_conv < e_conv_sing
+    # measured 6.8e-3 against 0.158 (0.043)
+    assert e_conv < 0.08 * e_conv_sing
 
 
 def _cesaro(fsslib):
@@ -216,7 +219,8 @@ def test_sal_treecode_learned_kernel(fsslib, sal):
     sing = ctx.sal_tree(*d, kind=1).cpu().numpy()
     e_sing = np.linalg.norm(sing - ref) / np.linalg.norm(ref)
     print("treecode rel-L2 learned", e_ml, "singular part only", e_sing)
-    assert e_ml < e_sing
+    # measured 1.16e-2 against 0.147 (0.079)
+    assert e_ml < 0.15 * e_sing
 
 
 def _random_network(seed):
