@@ -220,6 +220,54 @@ __device__ inline void bary_inv(const double *__restrict__ R, const double *p, d
 
 // modified charge q_k = sum_idx Vinv[idx][k] M_idx from the moments M, for
 // one k (one lane per charge)
+// Orders one wave's LDS accesses without a workgroup barrier: the LDS
+// executes the instructions of a wave in order, so a compiler fence at
+// wavefront scope suffices between a lane's store and another lane's load.
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sums M[0, P) over the lanes of every group of G lanes (G = 64: the wave;
+// G = 32: each half) through LDS, eight values at a time and in a fixed
+// order (8 rows per partial, then the G / 8 partials), and leaves the sums in
+// every lane of the group.  In place of a butterfly of P x log2(G) shuffles
+// (two ds_bpermute each for a double, every one waited for).  buf: this
+// wave's LDS slice of UP_LDS<P, G> doubles.
+template <int P, int G> constexpr int UP_LDS = 64 * 8 + 64 + (64 / G) * P;
+template <int P, int G>
+__device__ inline void group_sum(double *M, double *buf)
+{
+    constexpr int NB = G / 8; // blocks of 8 rows per group
+    const int lane = threadIdx.x & 63, grp = lane / G, gl = lane % G;
+    const int v = gl & 7, b = gl >> 3;
+    double *rows = buf;          // [64][8]
+    double *part = buf + 64 * 8; // [64]
+    double *tot = part + 64;     // [64 / G][P]
+#pragma unroll
+    for (int c = 0; c < (P + 7) / 8; c++) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) rows[lane * 8 + i] = 8 * c + i < P ? M[8 * c + i] : 0.0;
+        wave_sync();
+        double sp = 0.0;
+#pragma unroll
+        for (int r = 0; r < 8; r++) sp += rows[(grp * G + b * 8 + r) * 8 + v];
+        part[grp * G + gl] = sp;
+        wave_sync();
+        if (gl < 8 && 8 * c + gl < P) {
+            double t = 0.0;
+#pragma unroll
+            for (int bb = 0; bb < NB; bb++) t += part[grp * G + bb * 8 + gl];
+            tot[grp * P + 8 * c + gl] = t;
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int k = 0; k < P; k++) M[k] = tot[grp * P + k];
+}
+
 template <typename Real, int D>
 __device__ inline void write_charge(const double *M, const double *__restrict__ vinv, int64_t p0, int32_t g,
                                     int k, typename Rec<Real>::T4 *src)
@@ -271,9 +319,8 @@ __global__ void __launch_bounds__(256) k_upward(const int4 *__restrict__ items, 
 #pragma unroll
         for (int k = 0; k < P; k++) M[k] = fma(m[k], w, M[k]);
     }
-#pragma unroll
-    for (int k = 0; k < P; k++)
-        for (int off = 32; off > 0; off >>= 1) M[k] += __shfl_xor(M[k], off, 64);
+    __shared__ double lds_up[4][UP_LDS<P, 64>];
+    group_sum<P, 64>(M, lds_up[threadIdx.x >> 6]);
     // every lane holds M: lane k < P writes charge k (or partial moment k)
     if (lane >= P) return;
     if (it.w < 0) {
@@ -324,9 +371,8 @@ __global__ void __launch_bounds__(256) k_upward_pair(const int4 *__restrict__ it
 #pragma unroll
         for (int k = 0; k < P; k++) M[k] = m[k] * w;
     }
-#pragma unroll
-    for (int k = 0; k < P; k++)
-        for (int off = 16; off > 0; off >>= 1) M[k] += __shfl_xor(M[k], off, 64);
+    __shared__ double lds_up[4][UP_LDS<P, 32>];
+    group_sum<P, 32>(M, lds_up[threadIdx.x >> 6]);
     if (!mine) return;
     // 32 lanes per item: lane hl writes charges (or partial moments) hl and,
     // for P > 32 (degree 7 and 8), hl + 32
@@ -365,9 +411,8 @@ __global__ void __launch_bounds__(256) k_upward_finish(const int4 *__restrict__ 
 #pragma unroll
         for (int k = 0; k < P; k++) M[k] += o[k];
     }
-#pragma unroll
-    for (int k = 0; k < P; k++)
-        for (int off = 32; off > 0; off >>= 1) M[k] += __shfl_xor(M[k], off, 64);
+    __shared__ double lds_up[4][UP_LDS<P, 64>];
+    group_sum<P, 64>(M, lds_up[threadIdx.x >> 6]);
     if (lane < P) write_charge<Real, D>(M, vinv, p0, mn.x, lane, src);
 }
 
@@ -711,16 +756,6 @@ __device__ inline void far_tile(const float4 *yk, const float4 &s, const float *
         pq[1] = fmaf(c[q], d[q][1], pq[1]);
         pq[2] = fmaf(c[q], d[q][2], pq[2]);
     }
-}
-
-// Orders one wave's LDS accesses without a workgroup barrier: the LDS
-// executes the instructions of a wave in order, so a compiler fence at
-// wavefront scope suffices between a lane's store and another lane's load.
-__device__ inline void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Proxy values from the item's source list: one wave per item (four
