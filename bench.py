@@ -315,10 +315,11 @@ def run_batch(args, W, fss, torch):
     # assignment): NT masked self pairs are not algorithmic work
     alg_pairs = st.pairs_near - NT
     achieved = alg_pairs * FLOPS_PAIR_FP32 / (near_ms * 1e-3) / 1e12
-    roofline = dict(kernel=f"k_near<float,{tile},false>", bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS,
+    roofline = dict(kernel=f"k_near<float,{tile},false>", bound="mfma", engine="valu", achieved=achieved,
+                    peak=PEAK_FP32_TFLOPS,
                     unit="TFLOP/s", frac=achieved / PEAK_FP32_TFLOPS,
                     traffic=traffic_from_profiles("k_near32", "c4"), flops_per_launch=alg_pairs * FLOPS_PAIR_FP32,
-                    note=f"{alg_pairs} near pairs x {FLOPS_PAIR_FP32} flops / mean "
+                    note=f"compute bound (vector ALU): {alg_pairs} near pairs x {FLOPS_PAIR_FP32} flops / mean "
                          f"k_near<float,{tile},false> duration (HIP events on the context stream); traffic: "
                          "HBM bytes per launch from profiles/k_near32_c4_traffic.md")
     cpu = None
@@ -343,7 +344,8 @@ def run_batch(args, W, fss, torch):
         "metric": METRIC, "value": value, "unit": "velocities/s", "n_gpus": 1,
         "steps": W["rk4_steps"], "warmup": 1, "ms_per_step": t_rk * 1e3 / W["rk4_steps"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 kernels / f64 reductions", "data": "synthetic",
+        "dtype": "f32", "precision": "FP32 kernel evaluations; FP64 reductions, interpolation and RK4",
+        "data": "synthetic",
         "config": {"workload": (f"c4: {K} batched BLTC instances, N log-uniform in [2e4, 1e6] "
                                 f"(total {NT}), uniform random points, equal areas, 1-6 Gaussian "
                                 f"vortices, degree {W['degree']}, theta {W['theta']}, mixed precision, "
@@ -537,11 +539,12 @@ def run_single(args, W, fss, torch):
         units, flp, ms = head["pairs_far"], (FLOPS_PAIR_FP32 if mixed else FLOPS_PAIR_FP64), head["far_kernel_ms"]
         what = "proxy-source evaluations"
     achieved = units * flp / (ms * 1e-3) / 1e12
-    roofline = dict(kernel=kname, bound="valu", achieved=achieved, peak=peak, unit="TFLOP/s",
+    roofline = dict(kernel=kname, bound="mfma", engine="valu", achieved=achieved, peak=peak, unit="TFLOP/s",
                     frac=achieved / peak, traffic=traffic_from_profiles(kfile, W["name"]),
                     flops_per_launch=units * flp,
-                    note=f"{units} {what} x {flp} flops / mean {kname} duration {ms:.3f} ms (HIP events on "
-                         f"the context stream); traffic: HBM bytes per launch from "
+                    note=f"compute bound (bound 'mfma' = the compute peak; this kernel runs on the vector ALU, "
+                         f"no matrix instructions): {units} {what} x {flp} flops / mean {kname} duration "
+                         f"{ms:.3f} ms (HIP events on the context stream); traffic: HBM bytes per launch from "
                          f"profiles/{kfile}_{W['name']}_traffic.md")
 
     variants = {}
@@ -617,7 +620,9 @@ def run_single(args, W, fss, torch):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if not mixed else "f32 kernels / f64 reductions",
+        "dtype": "f64" if not mixed else "f32",
+        "precision": ("FP64 throughout" if not mixed else
+                      "FP32 kernel evaluations; FP64 reductions, interpolation and RK4"),
         "data": "synthetic",
         "config": {
             "workload": (f"{W['name']}: icosahedral-tree BLTC {W['headline']}, level-{W['level']} icosahedral "
