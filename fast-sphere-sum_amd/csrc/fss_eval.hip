@@ -421,11 +421,19 @@ __global__ void __launch_bounds__(256) k_upward_finish(const int4 *__restrict__ 
 // ------------------------------------------------------------------
 // Evaluates the staged entries [0, fill) owned by this lane (k = ph, ph + P,
 // ...) without branches in the loop body.
-template <typename Eval>
+template <int U = 1, typename Eval>
 __device__ inline void run_chunk(int fill, int ph, int P, bool act, Eval eval)
 {
     if (!act) return;
-    for (int k = ph; k < fill; k += P) eval(k);
+    // whole rounds of P slots (a wave-uniform count, so the loop control is
+    // scalar), then one more slot for the phases below fill mod P; U rounds
+    // per iteration where the registers allow it (the loads of the next
+    // source overlap the arithmetic of the current one)
+    const int rounds = fill / P;
+    int k = ph;
+#pragma unroll U
+    for (int i = 0; i < rounds; i++, k += P) eval(k);
+    if (k < fill) eval(k);
 }
 
 // Evaluates the staged entries [lo, fill) owned by this lane (the same slot to
@@ -485,6 +493,22 @@ template <int R> struct Acc<float, R> {
             }
     }
     __device__ float *part(int q) { return Pp[q]; }
+    // partials of the current chunk all finite (an infinite reciprocal of a
+    // zero denominator makes them infinite or NaN)
+    __device__ bool part_finite() const
+    {
+        bool ok = true;
+#pragma unroll
+        for (int q = 0; q < R; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) ok = ok && __builtin_isfinite(Pp[q][c]);
+        return ok;
+    }
+    __device__ void part_clear()
+    {
+#pragma unroll
+        for (int q = 0; q < R; q++) Pp[q][0] = Pp[q][1] = Pp[q][2] = 0.f;
+    }
     __device__ void flush()
     {
 #pragma unroll
@@ -510,7 +534,8 @@ template <int R> struct Acc<float, R> {
 // quarter-rate v_rcp_f64 and their Newton steps (relative error of each c
 // within a few ulp more than one reciprocal per pair); a zero denominator
 // (coincident points) falls back to one reciprocal per pair.  FP32: the same
-// with v_rcp_f32 (no Newton step) on the offset records.
+// with v_rcp_f32 (no Newton step) on the offset records, and the zero
+// denominator handled after the chunk instead of in the loop (below).
 template <int R, bool SPH, typename AccT>
 __device__ inline void near_tile(const double4 *xt, const double4 &s, AccT &A)
 {
@@ -543,6 +568,11 @@ __device__ inline void near_tile(const double4 *xt, const double4 &s, AccT &A)
         pq[2] = fma(c[q], d[q][2], pq[2]);
     }
 }
+// FP32 has no test in the loop: a zero denominator (coincident points, or a
+// product of denominators below the FP32 range) makes the shared reciprocal
+// infinite and the chunk's partials of that lane infinite or NaN; k_near then
+// evaluates the lane's chunk again with one reciprocal per pair
+// (near_tile_pairwise), as a per-pair evaluation would have done throughout.
 template <int R, bool SPH, typename AccT>
 __device__ inline void near_tile(const float4 *xt, const float4 &s, AccT &A)
 {
@@ -555,24 +585,32 @@ __device__ inline void near_tile(const float4 *xt, const float4 &s, AccT &A)
         den[q] = fmaf(d[q][0], d[q][0], fmaf(d[q][1], d[q][1], d[q][2] * d[q][2]));
         pre[q] = q == 0 ? den[0] : pre[q - 1] * den[q];
     }
-    if (pre[R - 1] != 0.f) {
-        float rw = s.w * rcp32(pre[R - 1]);
+    float rw = s.w * rcp32(pre[R - 1]);
 #pragma unroll
-        for (int q = R - 1; q > 0; q--) {
-            c[q] = pre[q - 1] * rw;
-            rw = den[q] * rw;
-        }
-        c[0] = rw;
-    } else {
-#pragma unroll
-        for (int q = 0; q < R; q++) c[q] = s.w * rcp32(den[q]);
+    for (int q = R - 1; q > 0; q--) {
+        c[q] = pre[q - 1] * rw;
+        rw = den[q] * rw;
     }
+    c[0] = rw;
 #pragma unroll
     for (int q = 0; q < R; q++) {
         float *pq = A.part(q);
         pq[0] = fmaf(c[q], d[q][0], pq[0]);
         pq[1] = fmaf(c[q], d[q][1], pq[1]);
         pq[2] = fmaf(c[q], d[q][2], pq[2]);
+    }
+}
+template <int R, typename AccT>
+__device__ inline void near_tile_pairwise(const float4 *xt, const float4 &s, AccT &A)
+{
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        const float d0 = s.x - xt[q].x, d1 = s.y - xt[q].y, d2 = s.z - xt[q].z;
+        const float c = s.w * rcp32(fmaf(d0, d0, fmaf(d1, d1, d2 * d2)));
+        float *pq = A.part(q);
+        pq[0] = fmaf(c, d0, pq[0]);
+        pq[1] = fmaf(c, d1, pq[1]);
+        pq[2] = fmaf(c, d2, pq[2]);
     }
 }
 
@@ -656,7 +694,13 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
         }
         __syncthreads();
         const int mu = min(m, max(0, n0 - base)); // slots [0, mu) other, [mu, m) self sources
-        run_chunk(mu, ph, P, act, eval);
+        run_chunk<sizeof(Real) == 4 ? 2 : 1>(mu, ph, P, act, eval);
+        if constexpr (sizeof(Real) == 4) {
+            if (!A.part_finite()) { // a zero denominator among the lane's pairs
+                A.part_clear();
+                run_chunk(mu, ph, P, act, [&](int k) { near_tile_pairwise<R>(xt, lds[k], A); });
+            }
+        }
         if (mu < m) run_chunk_from(mu, m, ph, P, act, eval_masked);
         A.flush();
         __syncthreads();

@@ -225,3 +225,76 @@ def test_inputs_the_library_refuses(fsslib):
         ctx.velocity_tree(*half)
     got = _host(*ctx.velocity_tree(*d))
     assert all(np.all(np.isfinite(g)) for g in got)
+
+
+def _coincident_pair(orc, x, y, z, L):
+    """Two points 1e-11 of the leaf size from a leaf vertex v, towards the
+    centres of two leaves A and B that share v, such that each one's FP32
+    offset in the other's leaf frame equals the other's own offset (the
+    library's FP32 records: offset from the record's node centre, rounded;
+    moved to a target node's frame as (c_B - c_A) + offset in FP64, rounded
+    again).  The oracle's tree gives the same centres as the library's
+    (bit-exact, tests/test_gpu_parity.py)."""
+    from collections import defaultdict
+    T = orc.Tree(x, y, z, L, theta=0.7, many_count=10, degree=3, assign=0, eval_mode=1, node_rule=1)
+    info = T.tri_info()
+    lo, nleaf = orc.level_offset(L - 1), 20 * 4 ** (L - 1)
+    vx = T.verts()[T.tri_verts()][lo:lo + nleaf]
+    share = defaultdict(list)
+    for k in range(nleaf):
+        for c in range(3):
+            share[tuple(vx[k, c])].append(k)
+    for v, leaves in share.items():
+        v = np.array(v)
+        for a in range(len(leaves)):
+            for b in range(a + 1, len(leaves)):
+                A, B = leaves[a], leaves[b]
+                cA, cB = info[lo + A, :3], info[lo + B, :3]
+                p1 = v + 1e-11 * (cA - v)
+                p2 = v + 1e-11 * (cB - v)
+                p1, p2 = p1 / np.linalg.norm(p1), p2 / np.linalg.norm(p2)
+                o1, o2 = (p1 - cA).astype(np.float32), (p2 - cB).astype(np.float32)
+                if (np.all(((cB - cA) + o2.astype(np.float64)).astype(np.float32) == o1)
+                        and np.all(((cA - cB) + o1.astype(np.float64)).astype(np.float32) == o2)):
+                    return A, B, p1, p2, v
+    raise AssertionError("no coincident pair found")
+
+
+def test_mixed_zero_denominator_in_a_shared_reciprocal(fsslib, orc):
+    """The FP32 near field shares one reciprocal among a lane's targets; a
+    pair whose FP32 difference is exactly 0 makes it infinite.  Two
+    particles about 1e-12 apart on either side of a leaf corner (level-5
+    grid, its vertex there taken out) have that pair
+    in each other's near list: their own velocities are undefined in FP32
+    (NaN, like coincident points in the reference's kernel), and every other
+    target -- including the ones sharing their lanes -- stays finite and
+    equal to FP64 (the chunk is evaluated again with one reciprocal per
+    pair)."""
+    L = 4
+    x, y, z, tris = fsslib.grid_icosahedral(5)
+    A, B, p1, p2, v = _coincident_pair(orc, x, y, z, L)
+    # the grid vertex at v (a corner of the leaves) makes way for the pair
+    keep = np.linalg.norm(np.stack([x, y, z], 1) - v, axis=1) > 1e-9
+    assert keep.sum() == len(x) - 1
+    x = np.concatenate([x[keep], [p1[0], p2[0]]])
+    y = np.concatenate([y[keep], [p1[1], p2[1]]])
+    z = np.concatenate([z[keep], [p1[2], p2[2]]])
+    n = len(x)
+    vort, _ = fsslib.rossby_haurwitz(x, y, z, 0.5, 0.5)
+    area = np.full(n, 4 * np.pi / n)
+    d = _dev(x, y, z, vort, area)
+    cfg = dict(tree_levels=L, degree=3, theta=0.7, assign_mode=0, eval_mode=1, node_rule=1)
+    c32 = fsslib.Context(precision=fsslib.MIXED, **cfg)
+    c32.tree_build(*d[:3])
+    loc = c32.tree_export()["loc"][L - 1]
+    assert (loc[n - 2], loc[n - 1]) == (A, B)
+    c64 = fsslib.Context(**cfg)
+    c64.tree_build(*d[:3])
+    a = _host(*c32.velocity_tree(*d))
+    b = _host(*c64.velocity_tree(*d))
+    bad = ~np.all(np.isfinite(np.stack(a, 1)), 1)
+    print(f"leaves {A}, {B}: non-finite targets {np.nonzero(bad)[0]}")
+    assert list(np.nonzero(bad)[0]) == [n - 2, n - 1]
+    assert np.all(np.isfinite(np.stack(b, 1)))
+    keep = slice(0, n - 2)
+    assert _err([t[keep] for t in a], [t[keep] for t in b]) <= 1e-6
