@@ -624,7 +624,7 @@ __device__ inline void near_tile_pairwise(const float4 *xt, const float4 &s, Acc
 // stage their particle ids too and run a masked evaluation that skips the
 // self pair; the others run the branch-free loop.
 template <typename Real, int R, bool SPH>
-__global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
+__global__ void __launch_bounds__(64, sizeof(Real) == 4 && R == 2 ? 8 : 4) k_near(const NearItem *__restrict__ items,
                                              const int32_t *__restrict__ near_src,
                                              const typename Rec<Real>::T4 *__restrict__ src,
                                              const int32_t *__restrict__ recid,
@@ -681,15 +681,39 @@ __global__ void __launch_bounds__(64) k_near(const NearItem *__restrict__ items,
     };
     const int32_t *list = near_src + it.src_off;
     const int n0 = it.n_other, total = n0 + it.n_self;
+    constexpr int S = NCH / 64;
+    static_assert(S % 2 == 0, "staging batches");
     for (int base = 0; base < total; base += NCH) {
         const int m = min(NCH, total - base);
+        // the loads of a chunk in two batches of S / 2 slots, all of a batch
+        // issued before the first is used: the list entries (all S at once),
+        // then (clamped, so none is conditional) the records, their nodes and
+        // particle ids, then the node centres of to_frame
+        int32_t jj[S];
+        const bool self_part = base + m > n0; // the chunk holds self sources (wave-uniform)
 #pragma unroll
-        for (int s = 0; s < NCH / 64; s++) {
-            const int k = lane + 64 * s;
-            if (k < m) {
-                const int32_t j = list[base + k];
-                lds[k] = to_frame(src[j], src_node(j, p0, p, recnode), centres, ct);
-                if (base + k >= n0) lid[k] = recid[j];
+        for (int s = 0; s < S; s++) jj[s] = list[base + min(lane + 64 * s, m - 1)];
+#pragma unroll
+        for (int h = 0; h < S; h += S / 2) {
+            int32_t nd[S / 2], id[S / 2];
+            T4 rec[S / 2];
+#pragma unroll
+            for (int s = 0; s < S / 2; s++) {
+                const int32_t j = jj[h + s];
+                rec[s] = src[j];
+                const int32_t rn = recnode[min((int64_t)j, p0 - 1)];
+                nd[s] = j < p0 ? rn : (int32_t)((uint32_t)(j - p0) / (uint32_t)p);
+                id[s] = self_part ? recid[min((int64_t)j, p0 - 1)] : 0;
+            }
+#pragma unroll
+            for (int s = 0; s < S / 2; s++) rec[s] = to_frame(rec[s], nd[s], centres, ct);
+#pragma unroll
+            for (int s = 0; s < S / 2; s++) {
+                const int k = lane + 64 * (h + s);
+                if (k < m) {
+                    lds[k] = rec[s];
+                    if (base + k >= n0) lid[k] = id[s];
+                }
             }
         }
         __syncthreads();
