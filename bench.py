@@ -10,7 +10,8 @@ icosahedral-tree barycentric-Lagrange treecode (BLTC) in mixed precision
 
 `value` is measured where the metric's accuracy bar holds: the library's
 autotuner (fss_autotune) times candidate points (degree, theta, depth,
-many_count, near-kernel targets per lane) and keeps the fastest whose rel-L2
+many_count, near-kernel targets per lane, interpolation node set) and keeps
+the fastest whose rel-L2
 against the FP64 direct sum on 1e4 seeded targets is <= 1e-6; the line
 carries every candidate's time and error, and "meets_accuracy_bar".  The same JSON line carries the FP64 treecode at that
 point, the workload's own parameters (degree 3, theta 0.6: mixed and FP64,
@@ -42,6 +43,7 @@ FLOPS_PAIR_FP64_SPH = 16    # treecode near pairs: d (3) + |d|^2 (5) + division 
 FLOPS_PAIR_FP32 = 16        # treecode pairs: d (3) + |d|^2 (5) + rcp (1) + w* (1) + 3 FMA (6); 2w is in the record
 FLOPS_PAIR_FP32_DIRECT = 17  # direct sum: the same plus the x 1/2 of |d|^2 / 2
 BOX_CPU_SHARE = 16          # host cores a one-GPU job may use on the GPU box
+NODE_RULES = {0: "reference", 1: "sin^2", 2: "warp-and-blend"}
 
 
 def log(*a):
@@ -56,18 +58,21 @@ def workload(name):
     if name == "c3":
         # the accuracy point is picked by the library's autotuner among
         # candidates on the measured cost/accuracy frontier of this workload
-        # (degree 5-8, theta 0.3-0.6, 8 levels, many_count 10-64, two or four
-        # targets per lane in the near kernel)
-        cands = [dict(degree=5, theta=0.3, tree_levels=8, many_count=10),
-                 dict(degree=7, theta=0.5, tree_levels=8, many_count=64),
-                 dict(degree=7, theta=0.52, tree_levels=8, many_count=64),
-                 dict(degree=7, theta=0.525, tree_levels=8, many_count=64),
-                 dict(degree=7, theta=0.525, tree_levels=8, many_count=64, near_tile=4),
-                 dict(degree=7, theta=0.53, tree_levels=8, many_count=64),
-                 dict(degree=8, theta=0.58, tree_levels=8, many_count=45),
-                 dict(degree=8, theta=0.58, tree_levels=8, many_count=64),
-                 dict(degree=8, theta=0.58, tree_levels=8, many_count=64, near_tile=4),
-                 dict(degree=8, theta=0.6, tree_levels=8, many_count=64)]
+        # (degree 5-8, theta 0.3-0.675, 8 levels, many_count 10-80, two or
+        # four targets per lane in the near kernel, sin^2 (1) or
+        # warp-and-blend (2) interpolation nodes)
+        S2, WB = 1, 2
+        cands = [dict(degree=5, theta=0.3, tree_levels=8, many_count=10, node_rule=S2),
+                 dict(degree=7, theta=0.525, tree_levels=8, many_count=64, node_rule=S2),
+                 dict(degree=7, theta=0.525, tree_levels=8, many_count=64, near_tile=4, node_rule=S2),
+                 dict(degree=8, theta=0.58, tree_levels=8, many_count=64, node_rule=S2),
+                 dict(degree=7, theta=0.525, tree_levels=8, many_count=64, node_rule=WB),
+                 dict(degree=8, theta=0.65, tree_levels=8, many_count=64, node_rule=WB),
+                 dict(degree=8, theta=0.66, tree_levels=8, many_count=80, node_rule=WB),
+                 dict(degree=8, theta=0.67, tree_levels=8, many_count=64, node_rule=WB),
+                 dict(degree=8, theta=0.67, tree_levels=8, many_count=80, node_rule=WB),
+                 dict(degree=8, theta=0.67, tree_levels=8, many_count=80, near_tile=4, node_rule=WB),
+                 dict(degree=8, theta=0.675, tree_levels=8, many_count=80, node_rule=WB)]
         return dict(name="c3", level=10, field="gauss", targets=10000, seed=3033, headline="mixed",
                     config=dict(tree_levels=8, degree=3, theta=0.6, many_count=10),
                     metric=dict(tree_levels=8, degree=5, theta=0.3, many_count=10), candidates=cands)
@@ -185,7 +190,8 @@ def cpu_baseline(W, P, x, y, z, vort, area):
     cores = host_cores()
     t0 = time.perf_counter()
     T = oracle.Tree(x, y, z, P["tree_levels"], theta=P["theta"], many_count=P["many_count"],
-                    degree=P["degree"], assign=oracle.ASSIGN_ROBUST, eval_mode=oracle.EVAL_BLTC, node_rule=1)
+                    degree=P["degree"], assign=oracle.ASSIGN_ROBUST, eval_mode=oracle.EVAL_BLTC,
+                    node_rule=P.get("node_rule", 1))
     build = time.perf_counter() - t0
     sel, count, what = None, len(x), f"one full treecode stage at N={len(x)}"
     if len(x) > 2_000_000 and P["tree_levels"] >= 2:
@@ -474,7 +480,9 @@ def run_single(args, W, fss, torch):
     del ref_ctx
 
     def run_tree(P, precision, label, steps, eval_mode=fss.EVAL_BLTC, assign=fss.ASSIGN_ROBUST,
-                 rule=fss.NODES_SIN2):
+                 rule=None):
+        if rule is None:
+            rule = P.get("node_rule", fss.NODES_SIN2)
         ctx = fss.Context(tree_levels=P["tree_levels"], theta=P["theta"], degree=P["degree"],
                           many_count=P["many_count"], precision=precision, eval_mode=eval_mode,
                           assign_mode=assign, node_rule=rule)
@@ -627,11 +635,13 @@ def run_single(args, W, fss, torch):
         "config": {
             "workload": (f"{W['name']}: icosahedral-tree BLTC {W['headline']}, level-{W['level']} icosahedral "
                          f"vertices N={N}, {field}, dual-Voronoi areas, degree {HP['degree']}, theta {HP['theta']}, "
-                         f"{HP['tree_levels']} tree levels, many_count {HP['many_count']}"
+                         f"{HP['tree_levels']} tree levels, many_count {HP['many_count']}, "
+                         f"{NODE_RULES[HP.get('node_rule', 1)]} nodes"
                          + (f" (the accuracy point; the workload's own degree {W['config']['degree']}, theta "
                             f"{W['config']['theta']} in variants)" if W["metric"] else "")),
             "n": N, "tree_levels": HP["tree_levels"], "degree": HP["degree"], "theta": HP["theta"],
             "many_count": HP["many_count"], "near_tile": HP.get("near_tile") or (2 if mixed else 4),
+            "node_rule": NODE_RULES[HP.get("node_rule", 1)],
             "targets": W["targets"], "target_seed": W["seed"],
             "parallelism": "1 GPU",
         },
@@ -662,7 +672,7 @@ def main():
     ap.add_argument("--prec", choices=("fp64", "mixed"), default=None,
                     help="override the workload's headline precision (for profiling)")
     ap.add_argument("--point", default=None,
-                    help="headline treecode parameters 'degree,theta,tree_levels,many_count[,near_tile]' "
+                    help="headline treecode parameters 'degree,theta,tree_levels,many_count[,near_tile[,node_rule]]' "
                          "(skips the autotuner)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner: headline at the fixed accuracy point (degree 5, theta 0.3)")
@@ -699,6 +709,8 @@ def main():
         W["metric"] = dict(degree=int(f[0]), theta=float(f[1]), tree_levels=int(f[2]), many_count=int(f[3]))
         if len(f) > 4:
             W["metric"]["near_tile"] = int(f[4])
+        if len(f) > 5:
+            W["metric"]["node_rule"] = int(f[5])
         args.no_tune = True
     return run_single(args, W, fss, torch)
 

@@ -104,10 +104,87 @@ static void lu_solve(int n, const double *a, const int32_t *ipiv, double *b)
         }
 }
 
+// Legendre-Gauss-Lobatto points of order n on [-1, 1], ascending: the
+// endpoints and the roots of P_n', by Newton steps on x P_n - P_{n-1} from
+// the Chebyshev-Gauss-Lobatto points (the iteration of Canuto, Hussaini,
+// Quarteroni and Zang, Spectral Methods, 2.3).
+static void gll_points(int n, double *x)
+{
+    for (int i = 0; i <= n; i++) x[i] = -std::cos(M_PI * i / n);
+    for (int it = 0; it < 100; it++) {
+        double dmax = 0.0;
+        for (int i = 1; i < n; i++) {
+            double p0 = 1.0, p1 = x[i];
+            for (int k = 2; k <= n; k++) {
+                const double p2 = ((2 * k - 1) * x[i] * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            const double dx = (x[i] * p1 - p0) / ((n + 1) * p1);
+            x[i] -= dx;
+            dmax = std::max(dmax, std::fabs(dx));
+        }
+        if (dmax < 1e-16) break;
+    }
+}
+
+// Warburton's warp-and-blend nodes (T. Warburton, An explicit construction of
+// interpolation nodes on the simplex, J. Eng. Math. 56 (2006) 247-262): the
+// equispaced lattice of the equilateral triangle moved along each edge
+// direction by the 1-D map from equispaced to Gauss-Lobatto points, blended
+// towards the interior, with the paper's optimised blending parameters.
+// Edge nodes are the Gauss-Lobatto points; Lebesgue constants 4.3 at degree
+// 7 and 5.0 at 8 against 10.2 and 15.6 of the sin^2 rule.
+static void warp_blend_nodes(int n, double *pts)
+{
+    static const double alpopt[15] = {0.0,    0.0,    1.4152, 0.1001, 0.2751, 0.9800, 1.0999, 1.2832,
+                                      1.3648, 1.4773, 1.4959, 1.5743, 1.5770, 1.6223, 1.6258};
+    const double alpha = n <= 15 ? alpopt[n - 1] : 5.0 / 3.0;
+    std::vector<double> lgl(n + 1), req(n + 1);
+    gll_points(n, lgl.data());
+    for (int i = 0; i <= n; i++) req[i] = -1.0 + 2.0 * i / n;
+    // warp(r): the Lagrange interpolant through the equispaced points of
+    // (lgl - req), divided by 1 - r^2 inside the interval
+    auto warp = [&](double r) {
+        double w = 0.0;
+        for (int i = 0; i <= n; i++) {
+            double l = 1.0;
+            for (int j = 0; j <= n; j++)
+                if (j != i) l *= (r - req[j]) / (req[i] - req[j]);
+            w += l * (lgl[i] - req[i]);
+        }
+        return std::fabs(r) < 1.0 - 1e-10 ? w / (1.0 - r * r) : 0.0;
+    };
+    const double s3 = std::sqrt(3.0);
+    int k = 0;
+    for (int a = 0; a <= n; a++)
+        for (int c = 0; c <= n - a; c++, k++) {
+            const double L1 = (double)a / n, L3 = (double)c / n, L2 = 1.0 - L1 - L3;
+            double x = -L2 + L3, y = (-L2 - L3 + 2.0 * L1) / s3;
+            const double w1 = 4 * L2 * L3 * warp(L3 - L2) * (1 + (alpha * L1) * (alpha * L1));
+            const double w2 = 4 * L1 * L3 * warp(L1 - L3) * (1 + (alpha * L2) * (alpha * L2));
+            const double w3 = 4 * L1 * L2 * warp(L2 - L1) * (1 + (alpha * L3) * (alpha * L3));
+            x += w1 + std::cos(2 * M_PI / 3) * w2 + std::cos(4 * M_PI / 3) * w3;
+            y += std::sin(2 * M_PI / 3) * w2 + std::sin(4 * M_PI / 3) * w3;
+            // barycentric coordinates in the triangle (-1, -1/sqrt3), (1, -1/sqrt3), (0, 2/sqrt3)
+            const double b3 = (s3 * y + 1.0) / 3.0;
+            const double b2 = (x + 1.0 - b3) / 2.0;
+            const double b1 = 1.0 - b2 - b3;
+            pts[3 * k + 0] = b1;
+            pts[3 * k + 1] = b2;
+            pts[3 * k + 2] = b3;
+        }
+}
+
 // fekete_init (interp_utils.h:12-31): rule 0 runs it as written with the
-// uninitialised b taken as 0; rule 1 is the intended b = 1 - a - c.
+// uninitialised b taken as 0; rule 1 is the intended b = 1 - a - c; rule 2
+// is the warp-and-blend set above.
 static void interp_nodes(int degree, int rule, double *pts)
 {
+    if (rule == FSS_NODES_WARP_BLEND) {
+        warp_blend_nodes(degree, pts);
+        return;
+    }
     double delta_x = 1.0 / degree;
     double a, b = 0.0, c;
     for (int i = 0; i < degree + 1; i++) {
@@ -194,7 +271,8 @@ static fss_status validate(const fss_config *c)
         return set_error("bad assign_mode"), FSS_ERR_INVALID;
     if (c->eval_mode != FSS_EVAL_PARITY && c->eval_mode != FSS_EVAL_BLTC)
         return set_error("bad eval_mode"), FSS_ERR_INVALID;
-    if (c->node_rule != FSS_NODES_REFERENCE && c->node_rule != FSS_NODES_SIN2)
+    if (c->node_rule != FSS_NODES_REFERENCE && c->node_rule != FSS_NODES_SIN2 &&
+        c->node_rule != FSS_NODES_WARP_BLEND)
         return set_error("bad node_rule"), FSS_ERR_INVALID;
     return FSS_OK;
 }

@@ -113,7 +113,8 @@ fss_status autotune(fss_ctx *ctx, const fss_particles *pp, const fss_tune_point 
     for (int k = 0; k < ncand; k++) {
         const fss_tune_point &c = cand[k];
         if (c.degree < 1 || c.degree > FSS_MAX_DEGREE || !(c.theta > 0) || c.tree_levels < 1 ||
-            c.tree_levels > 13 || c.many_count < 0 || (c.near_tile != 0 && c.near_tile != 2 && c.near_tile != 4))
+            c.tree_levels > 13 || c.many_count < 0 || (c.near_tile != 0 && c.near_tile != 2 && c.near_tile != 4) ||
+            c.node_rule < -1 || c.node_rule > FSS_NODES_WARP_BLEND)
             return set_error("autotune: candidate " + std::to_string(k) + " is out of range"), FSS_ERR_INVALID;
     }
     hipStream_t s = ctx->stream;
@@ -142,13 +143,15 @@ fss_status autotune(fss_ctx *ctx, const fss_particles *pp, const fss_tune_point 
     int pick = -1, most_accurate = 0;
     double pick_ms = std::numeric_limits<double>::infinity(), best_err = std::numeric_limits<double>::infinity();
     auto configure = [&](const fss_tune_point &c) -> fss_status {
-        const bool new_degree = c.degree != ctx->cfg.degree;
+        const int rule = c.node_rule >= 0 ? c.node_rule : keep.node_rule;
+        const bool new_interp = c.degree != ctx->cfg.degree || rule != ctx->cfg.node_rule;
         ctx->cfg.degree = c.degree;
+        ctx->cfg.node_rule = rule;
         ctx->cfg.theta = c.theta;
         ctx->cfg.tree_levels = c.tree_levels;
         ctx->cfg.many_count = c.many_count;
         ctx->near_r = c.near_tile;
-        if (new_degree) FSS_TRY(interp_setup(ctx));
+        if (new_interp) FSS_TRY(interp_setup(ctx));
         return tree_build(ctx, pp->x, pp->y, pp->z, n);
     };
     for (int k = 0; k < ncand; k++) {

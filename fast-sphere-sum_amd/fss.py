@@ -24,7 +24,7 @@ FSS_OK = 0
 FP64, MIXED = 0, 1
 ASSIGN_PARITY, ASSIGN_ROBUST = 0, 1
 EVAL_PARITY, EVAL_BLTC = 0, 1
-NODES_REFERENCE, NODES_SIN2 = 0, 1
+NODES_REFERENCE, NODES_SIN2, NODES_WARP_BLEND = 0, 1, 2
 
 _STATUS = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "out of memory",
            4: "invalid call order", 5: "numerical failure", 6: "no HIP device"}
@@ -54,7 +54,7 @@ class Velocity(C.Structure):
 
 class TunePoint(C.Structure):
     _fields_ = [("degree", C.c_int), ("theta", C.c_double), ("tree_levels", C.c_int), ("many_count", C.c_int),
-                ("near_tile", C.c_int)]
+                ("near_tile", C.c_int), ("node_rule", C.c_int)]
 
 
 class TreeStats(C.Structure):
@@ -369,7 +369,7 @@ class Context:
 
     def autotune(self, x, y, z, vort, area, candidates, idx, max_rel_l2, reps=3):
         """Fastest of `candidates` (dicts with degree, theta, tree_levels,
-        many_count, near_tile) whose rel-L2 against the FP64 direct sum at
+        many_count, near_tile, optional node_rule) whose rel-L2 against the FP64 direct sum at
         the targets idx is <= max_rel_l2; the context keeps it.  Returns
         (index or -1, per-candidate ms, per-candidate errors)."""
         import torch
@@ -377,7 +377,7 @@ class Context:
         arr = (TunePoint * len(candidates))()
         for k, c in enumerate(candidates):
             arr[k] = TunePoint(int(c["degree"]), float(c["theta"]), int(c["tree_levels"]),
-                               int(c["many_count"]), int(c.get("near_tile", 0)))
+                               int(c["many_count"]), int(c.get("near_tile", 0)), int(c.get("node_rule", -1)))
         idx = idx.to(device=self.device, dtype=torch.int64).contiguous()
         ms = np.zeros(len(candidates))
         err = np.zeros(len(candidates))
@@ -388,6 +388,8 @@ class Context:
         c = candidates[b if b >= 0 else int(np.argmin(err))]
         self.cfg.degree, self.cfg.theta = int(c["degree"]), float(c["theta"])
         self.cfg.tree_levels, self.cfg.many_count = int(c["tree_levels"]), int(c["many_count"])
+        if int(c.get("node_rule", -1)) >= 0:
+            self.cfg.node_rule = int(c["node_rule"])
         self.n = x.numel()
         return b, ms, err
 

@@ -56,8 +56,11 @@ typedef enum { FSS_EVAL_PARITY = 0, FSS_EVAL_BLTC = 1 } fss_eval_mode;
 /* Interpolation node rule (interp_utils.h:12-31).  REFERENCE runs fekete_init
  * as written with its uninitialised `b` taken as 0 (at degree 2 this is the
  * uniform 6-point lattice); SIN2 is the evident intended rule b = 1 - a - c
- * with each coordinate mapped by sin^2(pi x / 2) and normalised. */
-typedef enum { FSS_NODES_REFERENCE = 0, FSS_NODES_SIN2 = 1 } fss_node_rule;
+ * with each coordinate mapped by sin^2(pi x / 2) and normalised.  WARP_BLEND
+ * (no reference counterpart) is Warburton's warp-and-blend set (J. Eng. Math.
+ * 56 (2006) 247-262), whose Lebesgue constant at degree 7-8 is about a third
+ * of the sin^2 rule's. */
+typedef enum { FSS_NODES_REFERENCE = 0, FSS_NODES_SIN2 = 1, FSS_NODES_WARP_BLEND = 2 } fss_node_rule;
 
 /* Mirrors the run_information fields the velocity path reads
  * (struct_list.h:58,77-80,88-90; new_dynamics.cpp:43,78-79). */
@@ -167,6 +170,7 @@ typedef struct {
     int tree_levels;
     int many_count;
     int near_tile; /* targets per lane of the near kernel: 2, 4, or 0 (default) */
+    int node_rule; /* fss_node_rule of the interpolation, or -1 for the context's */
 } fss_tune_point;
 fss_status fss_autotune(fss_ctx *ctx, const fss_particles *p, const fss_tune_point *cand, int ncand,
                         const int64_t *idx, int64_t m, double max_rel_l2, int reps, int *best, double *ms,

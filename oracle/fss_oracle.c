@@ -38,7 +38,10 @@
  *                       against the MAC by brute force, partition invariant
  *                       sum n_t*n_s = N^2 (robust assignment)
  *   interpolation       degree-2 node set = uniform lattice, exact reproduction
- *                       of polynomials of degree <= d, basis independence
+ *                       of polynomials of degree <= d, basis independence;
+ *                       warp-and-blend nodes: symmetric under the six
+ *                       vertex permutations, Gauss-Lobatto points on every
+ *                       edge, Lebesgue constants of the published table
  *   tree sum            converges to the direct sum as theta -> 0 / degree up
  *   reference cp        converges with degree to the kernel at each target's
  *                       planar image c0 v1 + c1 v2 + (1 - c0 - c1) v3 (raw
@@ -622,12 +625,89 @@ ORC_API int64_t orc_tree_traverse(int tree_levels, const double *tri_info, const
 /* ------------------------------------------------------------------ */
 ORC_API int orc_interp_count(int degree) { return (degree + 1) * (degree + 2) / 2; }
 
+/* Gauss-Lobatto-Legendre points of order n on [-1, 1] (ascending): the
+ * endpoints and the n - 1 roots of P_n', each by Newton's method on
+ * f(x) = x P_n(x) - P_{n-1}(x), which vanishes exactly there for |x| < 1
+ * ((1 - x^2) P_n' = n (P_{n-1} - x P_n)), from the Chebyshev-Lobatto points. */
+static void gll_nodes(int n, double *x)
+{
+    x[0] = -1.0;
+    x[n] = 1.0;
+    for (int i = 1; i < n; i++) {
+        double r = -cos(M_PI * i / n);
+        for (int it = 0; it < 200; it++) {
+            double pm = 1.0, pn = r;
+            for (int k = 2; k <= n; k++) {
+                double pk = ((2.0 * k - 1.0) * r * pn - (k - 1.0) * pm) / k;
+                pm = pn;
+                pn = pk;
+            }
+            /* f' = P_n + x P_n' - P_{n-1}'; with P_n' and P_{n-1}' from the
+             * derivative identity this equals (n + 1) P_n */
+            double step = (r * pn - pm) / ((n + 1.0) * pn);
+            r -= step;
+            if (fabs(step) < 1e-17) break;
+        }
+        x[i] = r;
+    }
+}
+
+/* 1-D warp of Warburton's construction at r in [-1, 1]: the degree-n
+ * interpolant through the equispaced points r_i of the shifts (gll_i - r_i),
+ * divided by the edge blend 1 - r^2 (zero at the ends, where it vanishes). */
+static double wb_warp(int n, const double *gll, double r)
+{
+    double w = 0.0;
+    for (int i = 0; i <= n; i++) {
+        double ri = -1.0 + 2.0 * i / n, l = 1.0;
+        for (int j = 0; j <= n; j++) {
+            double rj = -1.0 + 2.0 * j / n;
+            if (j != i) l *= (r - rj) / (ri - rj);
+        }
+        w += l * (gll[i] - ri);
+    }
+    return (fabs(r) < 1.0 - 1e-10) ? w / (1.0 - r * r) : 0.0;
+}
+
 /* rule 0: fekete_init as written (interp_utils.h:12-31) with its uninitialised
  *         `b` taken as 0 (the value that yields the uniform lattice at degree 2);
  * rule 1: the evident intended rule, b = 1 - a - c, each coordinate mapped by
- *         sin^2(pi x / 2) and the triple normalised. */
+ *         sin^2(pi x / 2) and the triple normalised;
+ * rule 2: warp-and-blend nodes (no reference counterpart; T. Warburton,
+ *         J. Eng. Math. 56 (2006) 247-262, with the optimised blend exponents
+ *         alpha of Hesthaven & Warburton, Nodal Discontinuous Galerkin
+ *         Methods (2008), 6.1): the equilateral triangle's equispaced lattice
+ *         displaced along the three edge directions by blended 1-D warps. */
 ORC_API void orc_interp_nodes(int degree, int rule, double *pts)
 {
+    if (rule == 2) {
+        static const double alpha_opt[15] = {0.0000, 0.0000, 1.4152, 0.1001, 0.2751, 0.9800, 1.0999, 1.2832,
+                                             1.3648, 1.4773, 1.4959, 1.5743, 1.5770, 1.6223, 1.6258};
+        int n = degree;
+        double alpha = n <= 15 ? alpha_opt[n - 1] : 5.0 / 3.0;
+        double gll[64];
+        gll_nodes(n, gll);
+        int k = 0;
+        for (int i = 0; i <= n; i++)
+            for (int j = 0; j <= n - i; j++, k++) {
+                /* barycentric (l1, l2, l3) on the lattice; vertices of the
+                 * equilateral triangle: v1 = (-1, -1/sqrt3), v2 = (1, -1/sqrt3),
+                 * v3 = (0, 2/sqrt3) carry l2, l3, l1 */
+                double l1 = (double)i / n, l3 = (double)j / n, l2 = 1.0 - l1 - l3;
+                double px = l3 - l2, py = (2.0 * l1 - l2 - l3) / sqrt(3.0);
+                double d1 = 4.0 * l2 * l3 * wb_warp(n, gll, l3 - l2) * (1.0 + alpha * alpha * l1 * l1);
+                double d2 = 4.0 * l3 * l1 * wb_warp(n, gll, l1 - l3) * (1.0 + alpha * alpha * l2 * l2);
+                double d3 = 4.0 * l1 * l2 * wb_warp(n, gll, l2 - l1) * (1.0 + alpha * alpha * l3 * l3);
+                px += d1 - 0.5 * d2 - 0.5 * d3;
+                py += 0.5 * sqrt(3.0) * d2 - 0.5 * sqrt(3.0) * d3;
+                double b3 = (sqrt(3.0) * py + 1.0) / 3.0;
+                double b2 = 0.5 * (px + 1.0 - b3);
+                pts[3 * k + 0] = 1.0 - b2 - b3;
+                pts[3 * k + 1] = b2;
+                pts[3 * k + 2] = b3;
+            }
+        return;
+    }
     double delta_x = 1.0 / degree;
     double a, b = 0.0, c;
     for (int i = 0; i < degree + 1; i++) {

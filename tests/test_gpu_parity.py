@@ -197,7 +197,8 @@ def random_case(fsslib):
 @pytest.mark.parametrize("L,evalm,rule,degree,theta", [(5, 0, 0, 2, 0.7), (6, 0, 0, 2, 0.5),
                                                        (5, 1, 1, 2, 0.7), (5, 1, 1, 3, 0.6),
                                                        (4, 1, 1, 5, 0.5), (4, 1, 1, 7, 0.5),
-                                                       (4, 1, 1, 8, 0.5)])
+                                                       (4, 1, 1, 8, 0.5), (4, 1, 2, 7, 0.5),
+                                                       (4, 1, 2, 8, 0.5)])
 def test_tree_random_points_all_types(fsslib, orc, random_case, L, evalm, rule, degree, theta):
     """Uniform random points: uneven node counts, so pp, pc, cp and cc all occur."""
     x, y, z, vort, area = random_case
