@@ -9,9 +9,9 @@ icosahedral-tree barycentric-Lagrange treecode (BLTC) in mixed precision
 (FP32 kernel evaluations, FP64 reductions) with inputs resident in HBM.
 
 `value` is measured where the metric's accuracy bar holds: the library's
-autotuner (fss_autotune) times candidate points (degree, theta, depth,
-many_count, near-kernel targets per lane, interpolation node set) and keeps
-the fastest whose rel-L2
+autotuner (fss_autotune) times candidate points (degree, theta and its
+decay per level, depth, many_count, near-kernel targets per lane,
+interpolation node set) and keeps the fastest whose rel-L2
 against the FP64 direct sum on 1e4 seeded targets is <= 1e-6; the line
 carries every candidate's time and error, and "meets_accuracy_bar".  The same JSON line carries the FP64 treecode at that
 point, the workload's own parameters (degree 3, theta 0.6: mixed and FP64,
@@ -58,21 +58,26 @@ def workload(name):
     if name == "c3":
         # the accuracy point is picked by the library's autotuner among
         # candidates on the measured cost/accuracy frontier of this workload
-        # (degree 5-8, theta 0.3-0.675, 8 levels, many_count 10-80, two or
+        # (degree 5-8, leaf theta 0.3-1.1, 8 levels, many_count 10-80, a MAC
+        # tightened per level towards the root by theta_decay 0.85-1, two or
         # four targets per lane in the near kernel, sin^2 (1) or
         # warp-and-blend (2) interpolation nodes)
         S2, WB = 1, 2
         cands = [dict(degree=5, theta=0.3, tree_levels=8, many_count=10, node_rule=S2),
                  dict(degree=7, theta=0.525, tree_levels=8, many_count=64, node_rule=S2),
-                 dict(degree=7, theta=0.525, tree_levels=8, many_count=64, near_tile=4, node_rule=S2),
-                 dict(degree=8, theta=0.58, tree_levels=8, many_count=64, node_rule=S2),
-                 dict(degree=7, theta=0.525, tree_levels=8, many_count=64, node_rule=WB),
-                 dict(degree=8, theta=0.65, tree_levels=8, many_count=64, node_rule=WB),
-                 dict(degree=8, theta=0.66, tree_levels=8, many_count=80, node_rule=WB),
-                 dict(degree=8, theta=0.67, tree_levels=8, many_count=64, node_rule=WB),
                  dict(degree=8, theta=0.67, tree_levels=8, many_count=80, node_rule=WB),
                  dict(degree=8, theta=0.67, tree_levels=8, many_count=80, near_tile=4, node_rule=WB),
-                 dict(degree=8, theta=0.675, tree_levels=8, many_count=80, node_rule=WB)]
+                 dict(degree=6, theta=0.95, tree_levels=8, many_count=64, node_rule=WB, theta_decay=0.85),
+                 dict(degree=7, theta=0.9, tree_levels=8, many_count=80, node_rule=WB, theta_decay=0.92),
+                 dict(degree=7, theta=0.95, tree_levels=8, many_count=80, node_rule=WB, theta_decay=0.9),
+                 dict(degree=7, theta=1.0, tree_levels=8, many_count=64, node_rule=WB, theta_decay=0.9),
+                 dict(degree=7, theta=1.0, tree_levels=8, many_count=64, near_tile=4, node_rule=WB,
+                      theta_decay=0.9),
+                 dict(degree=7, theta=1.05, tree_levels=8, many_count=64, node_rule=WB, theta_decay=0.88),
+                 dict(degree=7, theta=1.05, tree_levels=8, many_count=80, node_rule=WB, theta_decay=0.9),
+                 dict(degree=7, theta=1.05, tree_levels=8, many_count=80, near_tile=4, node_rule=WB,
+                      theta_decay=0.9),
+                 dict(degree=7, theta=1.1, tree_levels=8, many_count=80, node_rule=WB, theta_decay=0.9)]
         return dict(name="c3", level=10, field="gauss", targets=10000, seed=3033, headline="mixed",
                     config=dict(tree_levels=8, degree=3, theta=0.6, many_count=10),
                     metric=dict(tree_levels=8, degree=5, theta=0.3, many_count=10), candidates=cands)
@@ -191,7 +196,7 @@ def cpu_baseline(W, P, x, y, z, vort, area):
     t0 = time.perf_counter()
     T = oracle.Tree(x, y, z, P["tree_levels"], theta=P["theta"], many_count=P["many_count"],
                     degree=P["degree"], assign=oracle.ASSIGN_ROBUST, eval_mode=oracle.EVAL_BLTC,
-                    node_rule=P.get("node_rule", 1))
+                    node_rule=P.get("node_rule", 1), theta_decay=P.get("theta_decay", 1.0))
     build = time.perf_counter() - t0
     sel, count, what = None, len(x), f"one full treecode stage at N={len(x)}"
     if len(x) > 2_000_000 and P["tree_levels"] >= 2:
@@ -485,7 +490,7 @@ def run_single(args, W, fss, torch):
             rule = P.get("node_rule", fss.NODES_SIN2)
         ctx = fss.Context(tree_levels=P["tree_levels"], theta=P["theta"], degree=P["degree"],
                           many_count=P["many_count"], precision=precision, eval_mode=eval_mode,
-                          assign_mode=assign, node_rule=rule)
+                          assign_mode=assign, node_rule=rule, theta_decay=P.get("theta_decay", 1.0))
         if P.get("near_tile"):
             ctx.set_near_tile(P["near_tile"])
         torch.cuda.synchronize()
@@ -636,12 +641,13 @@ def run_single(args, W, fss, torch):
             "workload": (f"{W['name']}: icosahedral-tree BLTC {W['headline']}, level-{W['level']} icosahedral "
                          f"vertices N={N}, {field}, dual-Voronoi areas, degree {HP['degree']}, theta {HP['theta']}, "
                          f"{HP['tree_levels']} tree levels, many_count {HP['many_count']}, "
+                         f"theta decay {HP.get('theta_decay', 1.0)} per level, "
                          f"{NODE_RULES[HP.get('node_rule', 1)]} nodes"
                          + (f" (the accuracy point; the workload's own degree {W['config']['degree']}, theta "
                             f"{W['config']['theta']} in variants)" if W["metric"] else "")),
             "n": N, "tree_levels": HP["tree_levels"], "degree": HP["degree"], "theta": HP["theta"],
             "many_count": HP["many_count"], "near_tile": HP.get("near_tile") or (2 if mixed else 4),
-            "node_rule": NODE_RULES[HP.get("node_rule", 1)],
+            "node_rule": NODE_RULES[HP.get("node_rule", 1)], "theta_decay": HP.get("theta_decay", 1.0),
             "targets": W["targets"], "target_seed": W["seed"],
             "parallelism": "1 GPU",
         },
@@ -672,7 +678,8 @@ def main():
     ap.add_argument("--prec", choices=("fp64", "mixed"), default=None,
                     help="override the workload's headline precision (for profiling)")
     ap.add_argument("--point", default=None,
-                    help="headline treecode parameters 'degree,theta,tree_levels,many_count[,near_tile[,node_rule]]' "
+                    help="headline treecode parameters "
+                         "'degree,theta,tree_levels,many_count[,near_tile[,node_rule[,theta_decay]]]' "
                          "(skips the autotuner)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner: headline at the fixed accuracy point (degree 5, theta 0.3)")
@@ -711,6 +718,8 @@ def main():
             W["metric"]["near_tile"] = int(f[4])
         if len(f) > 5:
             W["metric"]["node_rule"] = int(f[5])
+        if len(f) > 6:
+            W["metric"]["theta_decay"] = float(f[6])
         args.no_tune = True
     return run_single(args, W, fss, torch)
 

@@ -274,6 +274,8 @@ static fss_status validate(const fss_config *c)
     if (c->node_rule != FSS_NODES_REFERENCE && c->node_rule != FSS_NODES_SIN2 &&
         c->node_rule != FSS_NODES_WARP_BLEND)
         return set_error("bad node_rule"), FSS_ERR_INVALID;
+    if (!(c->theta_decay > 0 && c->theta_decay <= 1))
+        return set_error("theta_decay must be in (0, 1]"), FSS_ERR_INVALID;
     return FSS_OK;
 }
 
@@ -333,6 +335,7 @@ fss_status fss_config_default(fss_config *cfg)
     cfg->eval_mode = FSS_EVAL_PARITY;
     cfg->node_rule = FSS_NODES_REFERENCE;
     cfg->device = 0;
+    cfg->theta_decay = 1.0;  // the reference's single theta
     return FSS_OK;
 }
 

@@ -343,10 +343,39 @@ __device__ inline void mac_separation(const double *ti, const double *si, double
     }
 }
 
+// The MAC's theta per level of a pair's coarser node: theta at the leaf
+// level, times theta_decay per level towards the root (repeated products
+// from the leaves up, so host code forms the same doubles).
+struct MacTheta {
+    double t[13];
+};
+static MacTheta mac_thetas(int L, double theta, double decay)
+{
+    MacTheta m{};
+    m.t[L - 1] = theta;
+    for (int l = L - 2; l >= 0; l--) m.t[l] = m.t[l + 1] * decay;
+    return m;
+}
+
+// BLTC admits a pair only if the two nodes' caps are apart: the MAC compares
+// chord radii with the arc distance of the centres, so at theta near 1 it
+// admits triangles that share a vertex (their caps touch there), whose
+// corner proxies coincide and whose kernel is singular.  The caps' arc
+// radii 2 asin(r/2) must sum to less than the distance, with a relative
+// margin of 1e-12 that settles exact touching (centres and the shared vertex
+// on one great circle) the same way on every machine.  Below theta 0.98 at
+// every level this never removes a pair: the largest node (level 0) has
+// r = 0.641 and 2 asin(r/2) <= 1.02 r there, so the arc radii sum to less
+// than 1.02 x 0.98 d.
+__device__ inline bool caps_apart(const double *ti, const double *si, double distance)
+{
+    return 2.0 * asin(0.5 * ti[3]) + 2.0 * asin(0.5 * si[3]) < distance * (1.0 - 1e-12);
+}
+
 __global__ void k_trav_classify(const int4 *__restrict__ F, int64_t nF,
                                 const int32_t *__restrict__ counts,
-                                const double *__restrict__ tri_info, double theta, int many_count,
-                                int L, double radius, int8_t *act, int32_t *eflag, int32_t *scnt,
+                                const double *__restrict__ tri_info, MacTheta thetas, int many_count,
+                                int L, double radius, int bltc, int8_t *act, int32_t *eflag, int32_t *scnt,
                                 unsigned long long *margin_bits)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -355,6 +384,7 @@ __global__ void k_trav_classify(const int4 *__restrict__ F, int64_t nF,
     if (i < nF) {
         int4 q = F[i];
         int ct = q.x, cs = q.y, lt = q.z, ls = q.w;
+        const double theta = thetas.t[min(lt, ls)];
         int32_t nt = node_count(counts, lt, ct), ns = node_count(counts, ls, cs);
         const double *ti = tri_info + 4 * (level_offset(lt) + ct);
         const double *si = tri_info + 4 * (level_offset(ls) + cs);
@@ -372,6 +402,7 @@ __global__ void k_trav_classify(const int4 *__restrict__ F, int64_t nF,
     if (i >= nF) return;
     int4 q = F[i];
     int ct = q.x, cs = q.y, lt = q.z, ls = q.w;
+    const double theta = thetas.t[min(lt, ls)];
     int32_t nt = node_count(counts, lt, ct), ns = node_count(counts, ls, cs);
     int8_t a = ACT_SKIP;
     if (nt != 0 && ns != 0) {
@@ -379,7 +410,7 @@ __global__ void k_trav_classify(const int4 *__restrict__ F, int64_t nF,
         const double *si = tri_info + 4 * (level_offset(ls) + cs);
         double distance, separation;
         mac_separation(ti, si, radius, theta, distance, separation);
-        if ((distance > 0) && (separation < theta)) {
+        if ((distance > 0) && (separation < theta) && (!bltc || caps_apart(ti, si, distance))) {
             a = (int8_t)((nt > many_count ? 2 : 0) + (ns > many_count ? 1 : 0));
         } else if ((nt < many_count) && (ns < many_count)) {
             a = 0;
@@ -928,8 +959,9 @@ fss_status tree_build(fss_ctx *ctx, const double *x, const double *y, const doub
         FSS_TRY(cp.ensure(sizeof(int32_t) * nF));
         unsigned gb = (unsigned)div_up(nF, 256);
         hipLaunchKernelGGL(k_trav_classify, dim3(gb), dim3(256), 0, s, F0.as<int4>(), nF,
-                           T.counts.as<int32_t>(), T.tri_info.as<double>(), cfg.theta, cfg.many_count,
-                           L, cfg.radius, act.as<int8_t>(), ef.as<int32_t>(), sc.as<int32_t>(),
+                           T.counts.as<int32_t>(), T.tri_info.as<double>(),
+                           mac_thetas(L, cfg.theta, cfg.theta_decay), cfg.many_count,
+                           L, cfg.radius, bltc ? 1 : 0, act.as<int8_t>(), ef.as<int32_t>(), sc.as<int32_t>(),
                            margin.as<unsigned long long>());
         FSS_TRY(cub_exclusive_sum(S.cub, ef.as<int32_t>(), ep.as<int32_t>(), nF, s));
         FSS_TRY(cub_exclusive_sum(S.cub, sc.as<int32_t>(), cp.as<int32_t>(), nF, s));

@@ -114,7 +114,7 @@ fss_status autotune(fss_ctx *ctx, const fss_particles *pp, const fss_tune_point 
         const fss_tune_point &c = cand[k];
         if (c.degree < 1 || c.degree > FSS_MAX_DEGREE || !(c.theta > 0) || c.tree_levels < 1 ||
             c.tree_levels > 13 || c.many_count < 0 || (c.near_tile != 0 && c.near_tile != 2 && c.near_tile != 4) ||
-            c.node_rule < -1 || c.node_rule > FSS_NODES_WARP_BLEND)
+            c.node_rule < -1 || c.node_rule > FSS_NODES_WARP_BLEND || !(c.theta_decay >= 0 && c.theta_decay <= 1))
             return set_error("autotune: candidate " + std::to_string(k) + " is out of range"), FSS_ERR_INVALID;
     }
     hipStream_t s = ctx->stream;
@@ -148,6 +148,7 @@ fss_status autotune(fss_ctx *ctx, const fss_particles *pp, const fss_tune_point 
         ctx->cfg.degree = c.degree;
         ctx->cfg.node_rule = rule;
         ctx->cfg.theta = c.theta;
+        ctx->cfg.theta_decay = c.theta_decay > 0 ? c.theta_decay : keep.theta_decay;
         ctx->cfg.tree_levels = c.tree_levels;
         ctx->cfg.many_count = c.many_count;
         ctx->near_r = c.near_tile;

@@ -40,7 +40,7 @@ class Config(C.Structure):
     _fields_ = [("theta", C.c_double), ("degree", C.c_int), ("many_count", C.c_int),
                 ("tree_levels", C.c_int), ("radius", C.c_double), ("omega", C.c_double),
                 ("precision", C.c_int), ("assign_mode", C.c_int), ("eval_mode", C.c_int),
-                ("node_rule", C.c_int), ("device", C.c_int)]
+                ("node_rule", C.c_int), ("device", C.c_int), ("theta_decay", C.c_double)]
 
 
 class Particles(C.Structure):
@@ -54,7 +54,7 @@ class Velocity(C.Structure):
 
 class TunePoint(C.Structure):
     _fields_ = [("degree", C.c_int), ("theta", C.c_double), ("tree_levels", C.c_int), ("many_count", C.c_int),
-                ("near_tile", C.c_int), ("node_rule", C.c_int)]
+                ("near_tile", C.c_int), ("node_rule", C.c_int), ("theta_decay", C.c_double)]
 
 
 class TreeStats(C.Structure):
@@ -296,7 +296,7 @@ class Context:
 
     def __init__(self, theta=0.7, degree=2, many_count=10, tree_levels=1, omega=2 * np.pi,
                  precision=FP64, assign_mode=ASSIGN_PARITY, eval_mode=EVAL_PARITY,
-                 node_rule=NODES_REFERENCE, device=0, stream=None):
+                 node_rule=NODES_REFERENCE, device=0, stream=None, theta_decay=1.0):
         import torch
         L = lib()
         cfg = Config()
@@ -304,6 +304,7 @@ class Context:
         cfg.theta, cfg.degree, cfg.many_count, cfg.tree_levels = theta, degree, many_count, tree_levels
         cfg.omega, cfg.precision, cfg.assign_mode = omega, precision, assign_mode
         cfg.eval_mode, cfg.node_rule, cfg.device = eval_mode, node_rule, device
+        cfg.theta_decay = theta_decay
         self.cfg = cfg
         self.device = torch.device("cuda", device)
         h = C.c_void_p()
@@ -369,7 +370,7 @@ class Context:
 
     def autotune(self, x, y, z, vort, area, candidates, idx, max_rel_l2, reps=3):
         """Fastest of `candidates` (dicts with degree, theta, tree_levels,
-        many_count, near_tile, optional node_rule) whose rel-L2 against the FP64 direct sum at
+        many_count, near_tile, optional node_rule and theta_decay) whose rel-L2 against the FP64 direct sum at
         the targets idx is <= max_rel_l2; the context keeps it.  Returns
         (index or -1, per-candidate ms, per-candidate errors)."""
         import torch
@@ -377,7 +378,8 @@ class Context:
         arr = (TunePoint * len(candidates))()
         for k, c in enumerate(candidates):
             arr[k] = TunePoint(int(c["degree"]), float(c["theta"]), int(c["tree_levels"]),
-                               int(c["many_count"]), int(c.get("near_tile", 0)), int(c.get("node_rule", -1)))
+                               int(c["many_count"]), int(c.get("near_tile", 0)), int(c.get("node_rule", -1)),
+                               float(c.get("theta_decay", 0.0)))
         idx = idx.to(device=self.device, dtype=torch.int64).contiguous()
         ms = np.zeros(len(candidates))
         err = np.zeros(len(candidates))
@@ -390,6 +392,8 @@ class Context:
         self.cfg.tree_levels, self.cfg.many_count = int(c["tree_levels"]), int(c["many_count"])
         if int(c.get("node_rule", -1)) >= 0:
             self.cfg.node_rule = int(c["node_rule"])
+        if float(c.get("theta_decay", 0.0)) > 0:
+            self.cfg.theta_decay = float(c["theta_decay"])
         self.n = x.numel()
         return b, ms, err
 

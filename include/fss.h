@@ -76,6 +76,11 @@ typedef struct {
     int eval_mode;       /* fss_eval_mode */
     int node_rule;       /* fss_node_rule */
     int device;          /* HIP device ordinal */
+    double theta_decay;  /* level factor k of the MAC, 0 < k <= 1 (default 1: the reference's
+                            single theta): a pair whose coarser node is on level l is admissible
+                            when (r_t + r_s)/d < theta k^(tree_levels-1-l).  No reference
+                            counterpart: the error of an admitted interaction grows with the
+                            node size, so coarse levels get a tighter criterion. */
 } fss_config;
 
 typedef struct fss_ctx fss_ctx;
@@ -171,6 +176,7 @@ typedef struct {
     int many_count;
     int near_tile; /* targets per lane of the near kernel: 2, 4, or 0 (default) */
     int node_rule; /* fss_node_rule of the interpolation, or -1 for the context's */
+    double theta_decay; /* level factor of the MAC (fss_config.theta_decay), or 0 for the context's */
 } fss_tune_point;
 fss_status fss_autotune(fss_ctx *ctx, const fss_particles *p, const fss_tune_point *cand, int ncand,
                         const int64_t *idx, int64_t m, double max_rel_l2, int reps, int *best, double *ms,

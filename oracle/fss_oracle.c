@@ -547,11 +547,17 @@ enum { T_PP = 0, T_PC = 1, T_CP = 2, T_CC = 3 };
 
 /* out: 7 int32 per interaction {lev_t, lev_s, idx_t, idx_s, n_t, n_s, type}.
  * Returns the number of interactions, or -(needed) when cap is too small.
- * *min_margin receives min |sep - theta| over every MAC evaluated with d > 0. */
+ * *min_margin receives min |sep - theta| over every MAC evaluated with d > 0.
+ * theta_decay (1 for the reference): a pair whose coarser node is on level l
+ * is tested against theta_l = theta * decay^(tree_levels - 1 - l), formed by
+ * repeated products from the leaf level up (no reference counterpart). */
 ORC_API int64_t orc_tree_traverse(int tree_levels, const double *tri_info, const int32_t *counts,
-                                  double theta, int many_count, double radius, int32_t *out,
-                                  int64_t cap, double *min_margin)
+                                  double theta_leaf, double theta_decay, int many_count, double radius,
+                                  int caps_apart, int32_t *out, int64_t cap, double *min_margin)
 {
+    double theta_l[32];
+    theta_l[tree_levels - 1] = theta_leaf;
+    for (int l = tree_levels - 2; l >= 0; l--) theta_l[l] = theta_l[l + 1] * theta_decay;
     int64_t qcap = 1 << 16, qh = 0, qt = 0, nout = 0;
     int32_t *q = (int32_t *)malloc(sizeof(int32_t) * 4 * qcap);
     double margin = INFINITY;
@@ -578,6 +584,7 @@ ORC_API int64_t orc_tree_traverse(int tree_levels, const double *tri_info, const
         int64_t it = orc_level_offset(lt) + ct, is = orc_level_offset(ls) + cs;
         int32_t nt = counts[it], ns = counts[is];
         if (nt == 0 || ns == 0) continue;
+        double theta = theta_l[lt < ls ? lt : ls];
         const double *ti = tri_info + 4 * it, *si = tri_info + 4 * is;
         double distance = great_circ_dist(ti, si, radius);
         double separation = (ti[3] + si[3]) / distance;
@@ -590,7 +597,12 @@ ORC_API int64_t orc_tree_traverse(int tree_levels, const double *tri_info, const
             if (mg < margin) margin = mg;
         }
         int emit = -1;
-        if ((distance > 0) && (separation < theta)) {
+        /* BLTC only (no reference counterpart): the caps of arc radius
+         * 2 asin(r/2) must not touch (chord radii against an arc distance
+         * admit vertex-sharing triangles at theta near 1, whose corner
+         * proxies coincide) */
+        int apart = !caps_apart || 2.0 * asin(0.5 * ti[3]) + 2.0 * asin(0.5 * si[3]) < distance * (1.0 - 1e-12);
+        if ((distance > 0) && (separation < theta) && apart) {
             emit = (nt > many_count ? 2 : 0) + (ns > many_count ? 1 : 0);
             /* type string "<t><s>": pp=0 pc=1 cp=2 cc=3 */
         } else {
@@ -781,7 +793,7 @@ typedef struct {
     int32_t ipiv[64];
 } orc_tree;
 
-ORC_API void *orc_tree_create(int tree_levels, double theta, int many_count, int degree,
+ORC_API void *orc_tree_create(int tree_levels, double theta, double theta_decay, int many_count, int degree,
                               int assign_mode, int eval_mode, int node_rule, const double *x,
                               const double *y, const double *z, int64_t n)
 {
@@ -823,13 +835,13 @@ ORC_API void *orc_tree_create(int tree_levels, double theta, int many_count, int
     /* tree_traverse (fast_rhs.h:16-91) */
     int64_t cap = 1 << 20;
     t->inter = (int32_t *)malloc(sizeof(int32_t) * 7 * cap);
-    int64_t r = orc_tree_traverse(tree_levels, t->tri_info, t->counts, theta, many_count, 1.0,
-                                  t->inter, cap, &t->margin);
+    int64_t r = orc_tree_traverse(tree_levels, t->tri_info, t->counts, theta, theta_decay, many_count, 1.0,
+                                  eval_mode == EVAL_BLTC, t->inter, cap, &t->margin);
     if (r < 0) {
         cap = -r;
         t->inter = (int32_t *)realloc(t->inter, sizeof(int32_t) * 7 * cap);
-        r = orc_tree_traverse(tree_levels, t->tri_info, t->counts, theta, many_count, 1.0,
-                              t->inter, cap, &t->margin);
+        r = orc_tree_traverse(tree_levels, t->tri_info, t->counts, theta, theta_decay, many_count, 1.0,
+                              eval_mode == EVAL_BLTC, t->inter, cap, &t->margin);
     }
     t->ninter = r;
     return t;

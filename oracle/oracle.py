@@ -66,7 +66,7 @@ def load(path):
     L.orc_interp_init.restype = i32
     L.orc_interp_init.argtypes = [i32, i32, _dp, _dp, _ip]
     L.orc_tree_create.restype = vp
-    L.orc_tree_create.argtypes = [i32, f64, i32, i32, i32, i32, i32, _dp, _dp, _dp, i64]
+    L.orc_tree_create.argtypes = [i32, f64, f64, i32, i32, i32, i32, i32, _dp, _dp, _dp, i64]
     L.orc_tree_destroy.restype = None
     L.orc_tree_destroy.argtypes = [vp]
     for nm in ("orc_tree_ninter", "orc_tree_nverts"):
@@ -177,13 +177,13 @@ class Tree:
     (new_dynamics.cpp:76-94 call order) on particles x, y, z."""
 
     def __init__(self, x, y, z, tree_levels, theta=0.7, many_count=10, degree=2,
-                 assign=ASSIGN_PARITY, eval_mode=EVAL_PARITY, node_rule=0, library=None):
+                 assign=ASSIGN_PARITY, eval_mode=EVAL_PARITY, node_rule=0, library=None, theta_decay=1.0):
         L = self._L = library if library is not None else lib()
         self.x, self.y, self.z = _c(x), _c(y), _c(z)
         self.n = self.x.shape[0]
         self.L = tree_levels
         self.degree = degree
-        self.h = L.orc_tree_create(tree_levels, theta, many_count, degree, assign, eval_mode,
+        self.h = L.orc_tree_create(tree_levels, theta, theta_decay, many_count, degree, assign, eval_mode,
                                    node_rule, self.x, self.y, self.z, self.n)
         if not self.h:
             raise RuntimeError("orc_tree_create failed")

@@ -260,3 +260,50 @@ def test_tree_integer_data_bit_exact_level8(fsslib, orc, assign):
     # the minimising MAC test is the same pair on both sides (its distance may
     # differ in the last bit: libm acos on the host, ROCm acos on the device)
     assert abs(ctx.stats().mac_margin - T.mac_margin) <= 1e-14 and T.mac_margin > 1e-9
+
+
+@pytest.mark.parametrize("assign", [0, 1])
+def test_tree_level_theta_decay_matches_oracle(fsslib, orc, case5, assign):
+    """A MAC tightened by theta_decay per level towards the root (leaf theta
+    0.95, decay 0.85, warp-and-blend nodes, degree 4): the interaction list
+    bit-exact against the oracle, FP64 velocities within 1e-12, mixed within
+    1e-6 of FP64."""
+    x, y, z, vort, area, _ = case5
+    cfg = dict(tree_levels=6, theta=0.95, theta_decay=0.85, degree=4, assign_mode=assign, eval_mode=1,
+               node_rule=2)
+    print(f"theta decay N={len(x)} assign={assign}", flush=True)
+    d = _dev(x, y, z, vort, area)
+    ctx = fsslib.Context(**cfg)
+    ctx.tree_build(*d[:3])
+    T = orc.Tree(x, y, z, 6, theta=0.95, many_count=10, degree=4, assign=assign, eval_mode=1, node_rule=2,
+                 theta_decay=0.85)
+    inter = T.interactions()
+    np.testing.assert_array_equal(ctx.tree_export()["inter"], inter)
+    got = _host(*ctx.velocity_tree(*d))
+    eu, ew = _vel_err(got, T.rhs(vort, area, nthreads=8))
+    print("types", np.bincount(inter[:, 6], minlength=4), "rel-L2 vs oracle", eu, ew)
+    assert eu <= TOL_FP64 and ew <= TOL_FP64
+    c32 = fsslib.Context(precision=fsslib.MIXED, **cfg)
+    c32.tree_build(*d[:3])
+    assert _vel_err(_host(*c32.velocity_tree(*d)), got)[0] <= TOL_MIXED
+
+
+def test_tree_theta_one_keeps_touching_caps_apart(fsslib, orc):
+    """theta 1 with leaves above many_count (level-8 grid, 5 levels, 128
+    particles per leaf): the MAC alone admits leaf pairs that share a vertex,
+    whose corner proxies coincide (every velocity came out NaN); BLTC refuses
+    pairs whose caps touch, so the list equals the oracle's and every
+    velocity is finite, in both precisions."""
+    x, y, z, tris = fsslib.grid_icosahedral(8)
+    area = fsslib.voronoi_areas(x, y, z, tris)
+    vort = fsslib.gauss_vortices(x, y, z, area, 12, 2.0, 0.05, 0.3, seed=7)
+    d = _dev(x, y, z, vort, area)
+    print(f"theta 1 N={len(x)}", flush=True)
+    cfg = dict(tree_levels=5, theta=1.0, many_count=80, degree=5, assign_mode=1, eval_mode=1, node_rule=2)
+    T = orc.Tree(x, y, z, 5, theta=1.0, many_count=80, degree=5, assign=1, eval_mode=1, node_rule=2)
+    for prec in (fsslib.FP64, fsslib.MIXED):
+        ctx = fsslib.Context(precision=prec, **cfg)
+        ctx.tree_build(*d[:3])
+        np.testing.assert_array_equal(ctx.tree_export()["inter"], T.interactions())
+        got = _host(*ctx.velocity_tree(*d))
+        assert all(np.isfinite(g).all() for g in got)

@@ -182,11 +182,22 @@ def orc_off(l):
     return 20 * (4 ** np.asarray(l, dtype=np.int64) - 1) // 3
 
 
-@pytest.mark.parametrize("assign", [0, 1])
-def test_traversal_generation_order_and_mac(orc, grid5, assign):
+def _level_thetas(L, theta, decay):
+    """theta of each level of a pair's coarser node: theta at the leaves,
+    times decay per level towards the root."""
+    t = [0.0] * L
+    t[L - 1] = theta
+    for lv in range(L - 2, -1, -1):
+        t[lv] = t[lv + 1] * decay
+    return np.array(t)
+
+
+@pytest.mark.parametrize("assign,decay", [(0, 1.0), (1, 1.0), (0, 0.85), (1, 0.9)])
+def test_traversal_generation_order_and_mac(orc, grid5, assign, decay):
     x, y, z, vort, area, _ = grid5
-    L, theta, mc = 5, 0.7, 10
-    T = orc.Tree(x, y, z, L, theta=theta, many_count=mc, assign=assign)
+    L, theta0, mc = 5, 0.7 if decay == 1.0 else 0.95, 10
+    T = orc.Tree(x, y, z, L, theta=theta0, many_count=mc, assign=assign, theta_decay=decay)
+    theta = _level_thetas(L, theta0, decay)[np.minimum(T.interactions()[:, 0], T.interactions()[:, 1])]
     it = T.interactions()
     lt, ls, t, s, nt, ns, ty = it.T
     # FIFO order = generation order; a pair's generation is lev_t + lev_s
@@ -205,12 +216,45 @@ def test_traversal_generation_order_and_mac(orc, grid5, assign):
     assert ok[near].all()
 
 
-def test_traversal_covers_every_pair_once(orc, fsslib):
+def test_bltc_admits_only_pairs_whose_caps_are_apart(orc, grid5):
+    """At theta 1 the MAC (chord radii against the arc distance of the
+    centres) admits triangles that share a vertex, whose corner proxies then
+    coincide.  BLTC also requires the caps of arc radius 2 asin(r/2) to be
+    apart: every admitted pair passes, some pairs under theta are refused for
+    it, and the velocities stay finite; the reference dispatch (PARITY) keeps
+    the reference's list."""
+    x, y, z, vort, area, _ = grid5
+    L, th = 3, 1.0
+    T = orc.Tree(x, y, z, L, theta=th, many_count=10, degree=3, assign=1, eval_mode=1, node_rule=2)
+    it = T.interactions()
+    lt, ls, t, s_, nt, ns, ty = it.T
+    d, sep = _mac(T.tri_info(), lt, ls, t, s_, th)
+    ti, si = T.tri_info()[orc_off(lt) + t], T.tri_info()[orc_off(ls) + s_]
+    arcs = 2 * np.arcsin(ti[:, 3] / 2) + 2 * np.arcsin(si[:, 3] / 2)
+    admitted = ty != 0
+    assert admitted.any() and (arcs[admitted] < d[admitted]).all()
+    refused = (d > 0) & (sep < th) & (arcs >= d)
+    print("admitted", int(admitted.sum()), "refused for touching caps", int(refused.sum()))
+    assert refused.any() and (ty[refused] == 0).all()
+    u = np.stack(T.rhs(vort, area, nthreads=8)[:3], 1)
+    assert np.isfinite(u).all()
+    P = orc.Tree(x, y, z, L, theta=th, many_count=10, degree=3, assign=1, eval_mode=0, node_rule=1)
+    lt, ls, t, s_, nt, ns, ty = P.interactions().T
+    d, sep = _mac(P.tri_info(), lt, ls, t, s_, th)
+    ti, si = P.tri_info()[orc_off(lt) + t], P.tri_info()[orc_off(ls) + s_]
+    touching = (d > 0) & (sep < th) & (2 * np.arcsin(ti[:, 3] / 2) + 2 * np.arcsin(si[:, 3] / 2) >= d)
+    letters = (nt > 10) * 2 + (ns > 10)
+    assert (ty[touching] == letters[touching]).all() and (ty[touching] != 0).any()
+
+
+@pytest.mark.parametrize("decay", [1.0, 0.8])
+def test_traversal_covers_every_pair_once(orc, fsslib, decay):
     # small random set: every ordered particle pair (i, j) lies in exactly one
     # interaction (target node holds i, source node holds j)
     n = 400
     x, y, z = fsslib.grid_random(n, 3)
-    T = orc.Tree(x, y, z, 4, theta=0.7, many_count=3, assign=orc.ASSIGN_ROBUST)
+    T = orc.Tree(x, y, z, 4, theta=0.7 if decay == 1.0 else 0.9, many_count=3, assign=orc.ASSIGN_ROBUST,
+                 theta_decay=decay)
     cover = np.zeros((n, n), dtype=np.int32)
     for lt, ls, t, s, nt, ns, ty in T.interactions():
         a = T.node_list(lt, t)
@@ -284,6 +328,43 @@ def test_warp_blend_nodes(orc, degree):
     lam = _lebesgue(pts, degree)
     print(f"degree {degree}: Lebesgue {lam:.4f} (sin^2 rule {_lebesgue(orc.interp_nodes(degree, 1), degree):.2f})")
     assert abs(lam - want) < 0.006 * want
+
+
+def test_level_theta_decay_moves_the_error_off_the_coarse_levels(orc, fsslib):
+    """An admitted interaction's interpolation error grows with the node size
+    (about rho^(d+1) r / d^2 per unit source weight at separation ratio rho),
+    so with one theta the coarse levels dominate the error.  Tightening theta
+    by a factor per level towards the root (theta_decay) lowers the error at
+    the same leaf theta (0.7: 1.7e-5 -> 8.3e-7 with decay 0.85), and reaches
+    the accuracy of a single small theta with fewer direct pairs and fewer
+    proxy evaluations (theta 0.5: 2.7e-6, 26.6 M direct pairs, 24.6 M proxy
+    evaluations; leaf theta 0.9, decay 0.85: 3.0e-6, 15.9 M, 22.0 M): 20,000
+    random points, 5 levels, degree 6, warp-and-blend nodes."""
+    n = 20000
+    x, y, z = fsslib.grid_random(n, 31)
+    area = np.full(n, 4 * np.pi / n)
+    vort = fsslib.gauss_vortices(x, y, z, area, 6, 2.0, 0.05, 0.3, seed=6)
+    D = np.stack(orc.rhs_direct(x, y, z, vort, area, nthreads=8)[:3], 1)
+    p = 28
+
+    def run(theta, decay):
+        T = orc.Tree(x, y, z, 5, theta=theta, many_count=40, degree=6, assign=1, eval_mode=1, node_rule=2,
+                     theta_decay=decay)
+        U = np.stack(T.rhs(vort, area, nthreads=8)[:3], 1)
+        it = T.interactions().astype(np.int64)
+        ty = it[:, 6]
+        direct = int((it[ty == 0, 4] * it[ty == 0, 5]).sum())
+        proxy = int((it[ty == 1, 4] * p).sum() + (it[ty == 2, 5] * p).sum() + (ty == 3).sum() * p * p)
+        return np.linalg.norm(U - D) / np.linalg.norm(D), direct, proxy
+
+    e1, _, _ = run(0.7, 1.0)
+    e2, _, _ = run(0.7, 0.85)
+    ea, pa, fa = run(0.5, 1.0)
+    eb, pb, fb = run(0.9, 0.85)
+    print(f"theta 0.7: {e1:.3e}, with decay 0.85: {e2:.3e}; theta 0.5: {ea:.3e} ({pa} direct, {fa} proxy); "
+          f"leaf theta 0.9, decay 0.85: {eb:.3e} ({pb}, {fb})")
+    assert e2 < 0.1 * e1
+    assert eb < 1.25 * ea and pb < 0.7 * pa and fb < fa
 
 
 def test_warp_blend_nodes_halve_the_treecode_error(orc, fsslib):
