@@ -77,6 +77,8 @@ def workload(name):
                  dict(degree=7, theta=1.05, tree_levels=8, many_count=80, node_rule=WB, theta_decay=0.9),
                  dict(degree=7, theta=1.05, tree_levels=8, many_count=80, near_tile=4, node_rule=WB,
                       theta_decay=0.9),
+                 dict(degree=7, theta=1.1, tree_levels=8, many_count=80, node_rule=WB, theta_decay=0.89),
+                 dict(degree=7, theta=1.15, tree_levels=8, many_count=80, node_rule=WB, theta_decay=0.88),
                  dict(degree=7, theta=1.1, tree_levels=8, many_count=80, node_rule=WB, theta_decay=0.9)]
         return dict(name="c3", level=10, field="gauss", targets=10000, seed=3033, headline="mixed",
                     config=dict(tree_levels=8, degree=3, theta=0.6, many_count=10),
