@@ -87,8 +87,12 @@ def workload(name):
         return dict(name="c4", instances=64, nmin=2e4, nmax=1e6, degree=2, theta=0.7, many_count=10,
                     targets=10000, seed=4044, headline="mixed", rk4_steps=5, dt=0.005)
     if name == "c5":
-        return dict(name="c5", level=9, tree_levels=7, degree=3, theta=0.5, many_count=10, targets=1000,
-                    seed=5055, nmax=4000, bumps=20, headline="fp64")
+        # degree 3 with warp-and-blend nodes and the MAC tightened by 0.8 per
+        # level towards the root: 7.1e-3 against the direct sum with the same
+        # kernel in 19.7 ms, where one theta of 0.5 with sin^2 nodes gave
+        # 7.1e-2 in 20.9 ms
+        return dict(name="c5", level=9, tree_levels=7, degree=3, theta=0.6, many_count=10, targets=1000,
+                    seed=5055, nmax=4000, bumps=20, headline="fp64", node_rule=2, theta_decay=0.8)
     raise SystemExit(f"unknown config {name}")
 
 
@@ -404,7 +408,7 @@ def run_sal(args, W, fss, torch):
     dev = [torch.from_numpy(a).to("cuda:0") for a in (x, y, z, eta, area)]
     ctx = fss.Context(tree_levels=W["tree_levels"], theta=W["theta"], degree=W["degree"],
                       many_count=W["many_count"], eval_mode=fss.EVAL_BLTC, assign_mode=fss.ASSIGN_ROBUST,
-                      node_rule=fss.NODES_SIN2)
+                      node_rule=W["node_rule"], theta_decay=W["theta_decay"])
     ctx.sal_set_series(coef, table_points=1 << 22, A=A)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -454,7 +458,8 @@ def run_sal(args, W, fss, torch):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": (f"c5: SAL convolution by BLTC, level-9 icosahedral vertices N={N}, "
                                 f"{W['bumps']} Gaussian bumps, synthetic Love numbers n=1..{W['nmax']}, "
-                                f"{W['tree_levels']} tree levels, degree {W['degree']}, theta {W['theta']}, "
+                                f"{W['tree_levels']} tree levels, degree {W['degree']}, leaf theta {W['theta']}, "
+                                f"theta decay {W['theta_decay']} per level, warp-and-blend nodes, "
                                 "tabulated series kernel"), "n": N},
         "accuracy": {"rel_l2_vs_direct": e(g_tab), "targets": len(idx),
                      "note": "direct sum with the same (tabulated) kernel at sampled targets"},

@@ -611,7 +611,7 @@ fss_status fss_sal_set_series(fss_ctx *ctx, const double *c, int nmax, int64_t t
 
 fss_status fss_sal_kernel(fss_ctx *ctx, int kind, const double *s, int64_t m, double *g)
 {
-    if (!ctx || !s || !g || m < 1 || (kind != 0 && kind != 1)) return set_error("bad argument"), FSS_ERR_INVALID;
+    if (!ctx || !s || !g || m < 1 || kind < 0 || kind > 2) return set_error("bad argument"), FSS_ERR_INVALID;
     return sal_kernel_eval(ctx, kind, s, m, g);
 }
 

@@ -38,12 +38,22 @@ __global__ void k_series(const double *__restrict__ c, int N, const double *__re
     g[i] = legendre_sum(c, N, 1.0 - 2.0 * si * si);
 }
 
+// the two halves of the table (table_eval): t = cos(theta) = 1 - 2 s^2 on the
+// s-uniform half, 2 u^2 - 1 on the u-uniform half
 __global__ void k_series_table(const double *__restrict__ c, int N, int64_t M, double *tab)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
-    double si = (double)i / (double)(M - 1);
-    tab[i] = legendre_sum(c, N, 1.0 - 2.0 * si * si);
+    const int64_t m1 = M / 2;
+    double t;
+    if (i < m1) {
+        const double si = SAL_TAB_SPAN * (double)i / (double)(m1 - 1);
+        t = 1.0 - 2.0 * si * si;
+    } else {
+        const double ui = SAL_TAB_SPAN * (double)(i - m1) / (double)(M - m1 - 1);
+        t = 2.0 * ui * ui - 1.0;
+    }
+    tab[i] = legendre_sum(c, N, t);
 }
 
 // ------------------------------------------------------------------
@@ -445,7 +455,7 @@ __global__ void k_mul(const double *__restrict__ a, const double *__restrict__ b
 fss_status sal_set_series(fss_ctx *ctx, const double *c_host, int nmax, int64_t table_points, double A)
 {
     Sal &S = ctx->sal;
-    if (nmax < 1 || table_points < 4) return set_error("sal: nmax >= 1 and table_points >= 4"), FSS_ERR_INVALID;
+    if (nmax < 1 || table_points < 8) return set_error("sal: nmax >= 1 and table_points >= 8"), FSS_ERR_INVALID;
     hipStream_t s = ctx->stream;
     S.nmax = nmax;
     S.A = A;
@@ -613,11 +623,25 @@ fss_status sal_mlp_grad(fss_ctx *ctx, const float *x_host, const float *y_host, 
     return FSS_OK;
 }
 
+__global__ void k_table(const double *__restrict__ tab, int64_t M, const double *__restrict__ s, int64_t m,
+                        double *g)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) g[i] = table_eval(tab, M, s[i]);
+}
+
 fss_status sal_kernel_eval(fss_ctx *ctx, int kind, const double *s_dev, int64_t m, double *g_dev)
 {
     Sal &S = ctx->sal;
     hipStream_t s = ctx->stream;
     if (kind == 0) return sal_series_eval(ctx, s_dev, m, g_dev);
+    if (kind == 2) {
+        if (!S.c.p) return set_error("sal: series not set"), FSS_ERR_STATE;
+        hipLaunchKernelGGL(k_table, dim3((unsigned)div_up(m, 256)), dim3(256), 0, s, S.tab.as<double>(), S.M, s_dev,
+                           m, g_dev);
+        FSS_HIP(hipGetLastError());
+        return FSS_OK;
+    }
     if (!S.has_mlp) return set_error("sal: no network"), FSS_ERR_STATE;
     FSS_TRY(sal_mlp_prepare(ctx));
     hipLaunchKernelGGL(k_mlp_eval, dim3((unsigned)div_up(m, 256)), dim3(256), 0, s, S.mlpop.as<MlpOp>(), s_dev, m, S.A, S.mu, S.sigma, (float)std::log(S.s_min),

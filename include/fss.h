@@ -332,14 +332,18 @@ fss_status fss_sal_love_numbers(int nmax, uint64_t seed, double *h, double *k);
 /* c[0..nmax] (host), c[0] = 0. */
 fss_status fss_sal_coefficients(int nmax, const double *h, const double *k, double rho_ratio,
                                 int cesaro, double *c);
-/* Upload the series (host c) and tabulate it on table_points uniform s in
- * [0, 1]; A = 3 rho_ratio (1 - h'_inf)/(4 pi) scales the analytic singular part
+/* Upload the series (host c) and tabulate it on table_points points (>= 8):
+ * half uniform in s = sin(theta/2) over [0, 0.75], half uniform in
+ * u = cos(theta/2) over [0, 0.75] (read beyond theta = pi/2), so that the
+ * samples are as dense in theta at the antipode as elsewhere;
+ * A = 3 rho_ratio (1 - h'_inf)/(4 pi) scales the analytic singular part
  * A (1/(2s) - 1) of the learned kernel (sum_{n>=1} P_n = 1/(2 sin(theta/2)) - 1). */
 fss_status fss_sal_set_series(fss_ctx *ctx, const double *c, int nmax, int64_t table_points,
                               double A);
 /* G at m separations s (device arrays): kind 0 = series (Clenshaw, FP64),
  * kind 1 = learned kernel A (1/(2s) - 1) + mu + sigma * MLP(x), x as for
- * fss_sal_mlp_norm. */
+ * fss_sal_mlp_norm, kind 2 = the tabulated series (the kernel of
+ * fss_sal_tree's and fss_sal_direct's kind 0). */
 fss_status fss_sal_kernel(fss_ctx *ctx, int kind, const double *s, int64_t m, double *g);
 /* salpotential (GFnewtest.ipynb:9934-9949): out[r] = sum_j G(d(i_r, j)) eta_j A_j
  * including j = i (G at theta = 0 is the finite truncated series), for targets

@@ -69,6 +69,31 @@ def test_series_kernel_matches_oracle(sal):
     assert ratio.max() <= 4.0
 
 
+def test_table_kernel_matches_series_up_to_the_antipode(sal):
+    """The tabulated kernel (cubic interpolation, 2^22 samples) against the
+    device's Clenshaw series at the same s, from s = 1e-6 to the antipode.
+    The ringing of the truncated series has one wavelength in theta (about
+    8e-4 rad at n = 4000) everywhere, and ds/dtheta vanishes at the antipode:
+    a table uniform in s over all of [0, 1] missed it there (37% error at
+    s = 1, found as a 3.7e-3 gap between the table and series treecodes on the
+    level-9 grid), so the upper half of the table is uniform in
+    cos(theta/2).  Bar: 1e-8 of the kernel's scale max(|G|, 1) -- the cubic
+    interpolation error at this sampling is 5e-8 relative at s = 1e-7 and
+    below 3e-10 beyond 1e-5 (numpy emulation)."""
+    ctx, c, A, torch = sal
+    rng = np.random.default_rng(8)
+    s = np.concatenate([np.geomspace(1e-6, 1e-2, 400), rng.uniform(1e-2, 0.99, 1500),
+                        np.sort(rng.uniform(0.99, 1.0, 1500)), 1 - np.geomspace(1e-12, 1e-3, 300),
+                        [np.sqrt(0.5), 0.75, 1.0]])
+    st = torch.from_numpy(s)
+    tab = ctx.sal_kernel(st, kind=2).cpu().numpy()
+    ser = ctx.sal_kernel(st, kind=0).cpu().numpy()
+    rel = np.abs(tab - ser) / np.maximum(np.abs(ser), 1.0)
+    far = s > 0.99
+    print(f"max |table - series| / max(|G|, 1): {rel.max():.2e} (beyond s = 0.99: {rel[far].max():.2e})")
+    assert rel.max() <= 1e-8
+
+
 def test_sal_direct_table_matches_oracle(fsslib, sal):
     ctx, c, A, torch = sal
     x, y, z, _, area, _ = icos_case(fsslib, 5)
