@@ -176,17 +176,33 @@ def self_pairs(ctx):
     return int(it[same, 4].astype(np.int64).sum())
 
 
-def traffic_from_profiles(kernel, wl):
-    """HBM bytes per launch of `kernel` on workload `wl`, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a copy of
-    known size), if one exists."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{wl}_traffic.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1])
-    return None
+def plan_key(P, tile):
+    """The plan parameters of a treecode stage as one line, the form the
+    traffic summaries record theirs in (`traffic_plan:`)."""
+    return (f"degree={P['degree']} theta={P['theta']:g} levels={P['tree_levels']} many_count={P['many_count']} "
+            f"tile={tile} nodes={P.get('node_rule', 1)} decay={P.get('theta_decay', 1.0):g}")
+
+
+def traffic_from_profiles(kernel, wl, key):
+    """HBM bytes per launch of `kernel` on workload `wl` at the plan `key`,
+    from the committed rocprofv3 --pmc summaries (FETCH_SIZE and WRITE_SIZE
+    scaled by a copy of known size), and the summary's file name; (None,
+    None) when no summary was measured at that plan."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"{kernel}_{wl}*_traffic.md"))):
+        vals = {}
+        for line in open(path):
+            k, _, v = line.partition(":")
+            if k in ("traffic_plan", "traffic_bytes_per_launch"):
+                vals[k] = v.strip()
+        if vals.get("traffic_plan") == key and "traffic_bytes_per_launch" in vals:
+            return float(vals["traffic_bytes_per_launch"]), os.path.basename(path)
+    return None, None
+
+
+def traffic_note(fname, key):
+    return (f"HBM bytes per launch from profiles/{fname}" if fname else
+            f"null: no committed FETCH_SIZE/WRITE_SIZE summary at this plan ({key})")
 
 
 def cpu_baseline(W, P, x, y, z, vort, area):
@@ -332,13 +348,16 @@ def run_batch(args, W, fss, torch):
     # assignment): NT masked self pairs are not algorithmic work
     alg_pairs = st.pairs_near - NT
     achieved = alg_pairs * FLOPS_PAIR_FP32 / (near_ms * 1e-3) / 1e12
+    key = plan_key(dict(degree=W["degree"], theta=W["theta"], tree_levels=best, many_count=W["many_count"]),
+                   tile)
+    traffic, tfile = traffic_from_profiles("k_near32", "c4", key)
     roofline = dict(kernel=f"k_near<float,{tile},false>", bound="mfma", engine="valu", achieved=achieved,
                     peak=PEAK_FP32_TFLOPS,
                     unit="TFLOP/s", frac=achieved / PEAK_FP32_TFLOPS,
-                    traffic=traffic_from_profiles("k_near32", "c4"), flops_per_launch=alg_pairs * FLOPS_PAIR_FP32,
+                    traffic=traffic, flops_per_launch=alg_pairs * FLOPS_PAIR_FP32,
                     note=f"compute bound (vector ALU): {alg_pairs} near pairs x {FLOPS_PAIR_FP32} flops / mean "
                          f"k_near<float,{tile},false> duration (HIP events on the context stream); traffic: "
-                         "HBM bytes per launch from profiles/k_near32_c4_traffic.md")
+                         + traffic_note(tfile, key))
     cpu = None
     if not args.no_cpu:
         # the oracle's treecode stage (same BLTC parameters) on the tuning instance
@@ -546,8 +565,9 @@ def run_single(args, W, fss, torch):
     del ctx
     mixed = hprec == fss.MIXED
     peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
+    r = HP.get("near_tile") or (2 if mixed else 4)
+    key = plan_key(HP, r)
     if head["near_kernel_ms"] >= head["far_kernel_ms"]:
-        r = HP.get("near_tile") or (2 if mixed else 4)
         kname = f"k_near<float,{r},false>" if mixed else f"k_near<double,{r},true>"
         kfile = "k_near32" if mixed else "k_near64"
         units, flp, ms = head["pairs_near"] - spairs, (FLOPS_PAIR_FP32 if mixed else FLOPS_PAIR_FP64_SPH), \
@@ -560,12 +580,13 @@ def run_single(args, W, fss, torch):
         what = "proxy-source evaluations"
     achieved = units * flp / (ms * 1e-3) / 1e12
     roofline = dict(kernel=kname, bound="mfma", engine="valu", achieved=achieved, peak=peak, unit="TFLOP/s",
-                    frac=achieved / peak, traffic=traffic_from_profiles(kfile, W["name"]),
+                    frac=achieved / peak, traffic=None,
                     flops_per_launch=units * flp,
                     note=f"compute bound (bound 'mfma' = the compute peak; this kernel runs on the vector ALU, "
                          f"no matrix instructions): {units} {what} x {flp} flops / mean {kname} duration "
-                         f"{ms:.3f} ms (HIP events on the context stream); traffic: HBM bytes per launch from "
-                         f"profiles/{kfile}_{W['name']}_traffic.md")
+                         f"{ms:.3f} ms (HIP events on the context stream); traffic: ")
+    roofline["traffic"], tfile = traffic_from_profiles(kfile, W["name"], key)
+    roofline["note"] += traffic_note(tfile, key)
 
     variants = {}
     if not args.no_variants:

@@ -1040,6 +1040,27 @@ __device__ inline void add_interp(const double *__restrict__ tinv, const double 
     a2 += b2;
 }
 
+// interpolants of the nodes g (slots sl, negative: none) of the active lanes,
+// one pass per distinct node with its data read through scalar loads (per-lane
+// vector loads of the same 30+ doubles would load the address units, and
+// their registers push the kernel into spills)
+template <int D>
+__device__ inline void add_interp_wave(const double *__restrict__ tinv, const double *__restrict__ alpha,
+                                       int32_t g, int32_t sl, const double *pt, int normalise, double &a0,
+                                       double &a1, double &a2)
+{
+    bool todo = sl >= 0;
+    uint64_t m;
+    while ((m = __ballot(todo)) != 0) {
+        const int fl = __builtin_ctzll(m);
+        const int32_t gu = __builtin_amdgcn_readlane(g, fl), su = __builtin_amdgcn_readlane(sl, fl);
+        if (todo && g == gu) {
+            add_interp<D>(tinv, alpha, gu, su, pt, normalise, a0, a1, a2);
+            todo = false;
+        }
+    }
+}
+
 // One thread per main record, in tree order (neighbouring threads share
 // nodes, so node data and near-field slots are read once per wave and
 // coalesced): sums the particle's near-field results over its main path
@@ -1120,19 +1141,8 @@ __global__ void k_combine(const int32_t *__restrict__ recid, const int32_t *__re
                 const int32_t gu = __builtin_amdgcn_readfirstlane(cg), su = __builtin_amdgcn_readfirstlane(csl);
                 if (su >= 0) add_interp<D>(tinv, alpha, gu, su, pt, normalise, a0, a1, a2);
             } else {
-                // a few nodes per wave (the deep levels): one pass per distinct
-                // node, its data read through scalar loads (per-lane vector
-                // loads of the same 27+ doubles would load the address units)
-                bool todo = csl >= 0;
-                uint64_t m;
-                while ((m = __ballot(todo)) != 0) {
-                    const int fl = __builtin_ctzll(m);
-                    const int32_t gu = __builtin_amdgcn_readlane(cg, fl), su = __builtin_amdgcn_readlane(csl, fl);
-                    if (todo && cg == gu) {
-                        add_interp<D>(tinv, alpha, gu, su, pt, normalise, a0, a1, a2);
-                        todo = false;
-                    }
-                }
+                // a few nodes per wave (the deep levels)
+                add_interp_wave<D>(tinv, alpha, cg, csl, pt, normalise, a0, a1, a2);
             }
         }
     }
@@ -1148,8 +1158,7 @@ __global__ void k_combine(const int32_t *__restrict__ recid, const int32_t *__re
                 a1 += out[ostride + pos];
                 a2 += out[2 * ostride + pos];
             }
-            const int32_t sl = node_slot[g];
-            if (sl >= 0) add_interp<D>(tinv, alpha, g, sl, pt, normalise, a0, a1, a2);
+            add_interp_wave<D>(tinv, alpha, g, node_slot[g], pt, normalise, a0, a1, a2);
         }
     }
     const double s2 = a2 * scale;
