@@ -42,7 +42,7 @@ FLOPS_PAIR_FP64 = 15        # direct sum: 1 - x.y (5) + sub (1) + division (1) +
 FLOPS_PAIR_FP64_SPH = 16    # treecode near pairs: d (3) + |d|^2 (5) + division (1) + w* (1) + 3 FMA (6)
 FLOPS_PAIR_FP32 = 16        # treecode pairs: d (3) + |d|^2 (5) + rcp (1) + w* (1) + 3 FMA (6); 2w is in the record
 FLOPS_PAIR_FP32_DIRECT = 17  # direct sum: the same plus the x 1/2 of |d|^2 / 2
-BOX_CPU_SHARE = 16          # host cores a one-GPU job may use on the GPU box
+BOX_CPU_SHARE = 16          # host cores given to a one-GPU job (its share of the machine)
 NODE_RULES = {0: "reference", 1: "sin^2", 2: "warp-and-blend"}
 
 
