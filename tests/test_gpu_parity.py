@@ -165,6 +165,34 @@ def test_rk4_graph_replays_match_eager(fsslib, case4):
         np.testing.assert_array_equal(g, r)
 
 
+def test_rk4_graph_each_replay_matches_eager(fsslib, case4):
+    """One step per call, so every call after the first replays the captured
+    graph; between replays the same context and stream run other eager work
+    (a tree stage on another vorticity field, which rewrites the workspace the
+    graph reads) and the host synchronises.  Every replay equals the eager
+    step bit for bit."""
+    import torch
+
+    x, y, z, vort, area, _ = case4
+    dt = 0.01
+    ctx = fsslib.Context(tree_levels=4, assign_mode=1)
+    ref = fsslib.Context(tree_levels=4, assign_mode=1)
+    a = _dev(x, y, z, vort, area)
+    b = _dev(x, y, z, vort, area)
+    other = _dev(x, y, z, 2.0 * vort + 1.0, area)
+    ctx.tree_build(*b[:3])
+    ref.tree_build(*a[:3])
+    for k in range(4):
+        print(f"graph replay {k}", flush=True)
+        ref.rk4_step(*a, dt, use_tree=True)
+        ctx.rk4_run_graph(*b, dt, 1, use_tree=True)
+        torch.cuda.synchronize()
+        for g, r in zip(_host(*b[:4]), _host(*a[:4])):
+            np.testing.assert_array_equal(g, r)
+        ctx.velocity_tree(*other)
+        torch.cuda.synchronize()
+
+
 def test_rk4_graph_recaptured_after_workspace_grows(fsslib, case4, case5):
     """A captured graph holds the context's workspace addresses: a larger
     direct sum on the same context reallocates that workspace, and the next
