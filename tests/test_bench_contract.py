@@ -55,3 +55,15 @@ def test_traffic_lookup_matches_the_plan_exactly():
             if (kernel, wl, other) not in plans:
                 assert b.traffic_from_profiles(kernel, wl, other) == (None, None)
     assert b.traffic_note(None, "x").startswith("null")
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    import subprocess
+    import sys
+
+    for argv, env in ((["--gpus", "2"], {}), (["--gpus", "1"], {"WORLD_SIZE": "4"})):
+        r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py")] + argv, capture_output=True, text=True,
+                           env=dict(os.environ, **env), timeout=60)
+        assert r.returncode == 2
+        assert "one GPU only" in r.stderr
+        assert r.stdout == ""
